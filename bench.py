@@ -1,0 +1,206 @@
+"""Benchmark of the SLEEK HIP codec on one MI355X.
+
+Workload (default): the 512^3 float32 field of the lossy benchmark
+configuration (8 random sinusoids + diagonal ramp U[-1e3, 1e3] + N(0, 1e-3)
+noise, 0.01 % NaN/Inf/denormal/-0 values), resident in HBM.  One step
+compresses and then decompresses it in each of four modes: lossless and
+absolute error bounds {1e-2, 1e-3, 1e-4} x (max - min over finite values).
+
+Printed (one JSON line): whole-step throughput in GB/s (input bytes per
+direction / time, the reference's insize/t definition), per-kernel effective
+and algorithmic GB/s, the roofline entry of the dominant kernel (measured
+with HIP events on the launch stream), a hipMemcpy device-to-device
+comparator of the same bytes, and the reference CPU codec on the host cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, bytes/s
+MODES = (None, 1e-2, 1e-3, 1e-4)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=10)
+    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--scale", type=float, default=1.0, help="shrink every axis (tests)")
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    p.add_argument("--cpu-mb", type=int, default=64, help="CPU baseline sample size in MiB")
+    p.add_argument("--seed", type=int, default=1)
+    return p.parse_args()
+
+
+def cpu_baseline(a_np, mb):
+    """Reference CPU codec (oracle/_ref) if built, else the oracle restatement,
+    on a bounded leading sample, all host cores; lossless compress+decompress."""
+    from oracle import pyoracle as po
+
+    n = min(a_np.size, (mb << 20) // a_np.itemsize)
+    sample = np.ascontiguousarray(a_np[:n])
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    use_ref = po.ref() is not None
+    comp = (lambda x: po.ref_compress(x)) if use_ref else (lambda x: po.compress(x))
+    dec = (lambda s: po.ref_decompress(s, sample.dtype)) if use_ref else (lambda s: po.decompress(s, sample.dtype)[0])
+    s = comp(sample)
+    dec(s)  # warm-up
+    tc, td = [], []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        s = comp(sample)
+        t1 = time.perf_counter()
+        dec(s)
+        t2 = time.perf_counter()
+        tc.append(t1 - t0)
+        td.append(t2 - t1)
+    tcm, tdm = float(np.median(tc)), float(np.median(td))
+    return {
+        "value": round(2 * sample.nbytes / (tcm + tdm) / 1e9, 3),
+        "unit": "GB/s",
+        "cores": cores,
+        "kind": "reference" if use_ref else "port",
+        "sample": f"first {sample.nbytes >> 20} MiB of the same field, lossless compress+decompress, "
+                  f"median of 3 after 1 warm-up, OMP threads={os.environ.get('OMP_NUM_THREADS', cores)}",
+        "compress_GBps": round(sample.nbytes / tcm / 1e9, 3),
+        "decompress_GBps": round(sample.nbytes / tdm / 1e9, 3),
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this codec runs on one GPU; multi-GPU runs are out of scope "
+                                   "(the sharding design is described in DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+    import torch
+
+    from sleek_amd import codec
+    from sleek_amd.datagen import config_field, finite_range
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    x = config_field(1, seed=args.seed, device=dev, scale=args.scale).reshape(-1).contiguous()
+    nbytes = x.numel() * 4
+    rng = finite_range(x)
+    ebs = [None] + [float(np.float32(np.float32(rng) * np.float32(e))) for e in MODES[1:]]
+    ub = codec.upper_bound_size(nbytes)
+    stream = torch.cuda.current_stream()
+    bufs = [torch.empty(ub, dtype=torch.uint8, device=dev) for _ in ebs]
+    encs = [torch.zeros(1, dtype=torch.int64, device=dev) for _ in ebs]
+    decs = [torch.empty_like(x) for _ in ebs]
+    dsz = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws = codec.Workspace(nbytes, dev)
+
+    def step(events=None):
+        for i, eb in enumerate(ebs):
+            if events is not None:
+                events[i][0].record(stream)
+            codec.compress(x, eb, out=bufs[i], encsize=encs[i], workspace=ws)
+            if events is not None:
+                events[i][1].record(stream)
+            codec.decompress(bufs[i], torch.float32, eb, out=decs[i], decsize=dsz, workspace=ws)
+            if events is not None:
+                events[i][2].record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    # correctness guard of the timed configuration: lossless round trip is exact
+    assert torch.equal(decs[0].view(torch.int32), x.view(torch.int32)), "lossless round trip mismatch"
+    assert ws.status() == 0
+    enc_sizes = [int(e.item()) for e in encs]
+
+    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in ebs] for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for k in range(args.steps):
+        step(ev[k])
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_total = start.elapsed_time(end) / 1e3
+    ms_step = t_total / args.steps * 1e3
+
+    per = []
+    for i, eb in enumerate(ebs):
+        tc = np.median([ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(args.steps)]) / 1e3
+        td = np.median([ev[k][i][1].elapsed_time(ev[k][i][2]) for k in range(args.steps)]) / 1e3
+        alg = nbytes + enc_sizes[i]
+        per.append({
+            "mode": "lossless" if eb is None else f"eb={MODES[i]:g}*range",
+            "eb_abs": eb, "ratio": round(nbytes / enc_sizes[i], 4), "encsize": enc_sizes[i],
+            "compress_ms": round(tc * 1e3, 4), "decompress_ms": round(td * 1e3, 4),
+            "compress_GBps": round(nbytes / tc / 1e9, 1), "decompress_GBps": round(nbytes / td / 1e9, 1),
+            "compress_alg_GBps": round(alg / tc / 1e9, 1), "decompress_alg_GBps": round(alg / td / 1e9, 1),
+            "compress_hbm_frac": round(alg / tc / HBM_PEAK, 4), "decompress_hbm_frac": round(alg / td / HBM_PEAK, 4),
+        })
+
+    # dominant kernel: the launch with the largest median time
+    cands = []
+    for i, p in enumerate(per):
+        alg = nbytes + enc_sizes[i]
+        cands.append((p["compress_ms"], f"encode ({p['mode']})", alg))
+        cands.append((p["decompress_ms"], f"decode ({p['mode']})", alg))
+    tms, kname, alg = max(cands)
+    achieved = alg / (tms / 1e3) / 1e9
+
+    # hipMemcpy device-to-device comparator of the same bytes
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    mt = []
+    for _ in range(10):
+        a0 = torch.cuda.Event(enable_timing=True)
+        a1 = torch.cuda.Event(enable_timing=True)
+        a0.record(stream)
+        y.copy_(x)
+        a1.record(stream)
+        torch.cuda.synchronize()
+        mt.append(a0.elapsed_time(a1) / 1e3)
+    mtm = float(np.median(mt))
+
+    total_in = 2 * nbytes * len(ebs)
+    out = {
+        "metric": "compress+decompress GB/s (512^3 f32, lossless + 3 error bounds)",
+        "value": round(total_in / (t_total / args.steps) / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": "512^3 float32 sinusoid+ramp field with 0.01% specials; per step compress and "
+                               "decompress in lossless and eb={1e-2,1e-3,1e-4}*range modes",
+                   "shape": [int(round(512 * args.scale))] * 3, "bytes": nbytes},
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
+                     "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": round(tms, 4)},
+        "per_mode": per,
+        "memcpy_d2d": {"ms": round(mtm * 1e3, 4), "effective_GBps": round(nbytes / mtm / 1e9, 1),
+                       "traffic_GBps": round(2 * nbytes / mtm / 1e9, 1),
+                       "hbm_frac": round(2 * nbytes / mtm / HBM_PEAK, 4)},
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), args.cpu_mb)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

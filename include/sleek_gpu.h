@@ -1,0 +1,112 @@
+/*
+ * SLEEK for MI355X (gfx950): C ABI of the GPU codec.
+ *
+ * Plain C, no HIP or torch types: device pointers are void* / byte pointers,
+ * streams are passed as void* (a hipStream_t, or NULL for the null stream).
+ * Every function returns a status (SLEEK_OK or a negative SLEEK_E* code)
+ * unless noted.  The C++ drop-in headers sleek_lossless_gpu.h and
+ * sleek_lossy_gpu.h wrap these with the reference's overloads.
+ *
+ * Stream format: identical, byte for byte, to the reference codec's
+ * (int64 length header, uint16 per-chunk size table, byte-packed chunk
+ * payloads; 16 KiB chunks).
+ */
+#ifndef SLEEK_GPU_H
+#define SLEEK_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SLEEK_OK 0
+#define SLEEK_EINVAL (-1)      /* bad argument */
+#define SLEEK_EHIP (-2)        /* HIP runtime error (see sleek_gpu_last_error) */
+#define SLEEK_ECORRUPT (-3)    /* malformed stream detected while decoding */
+#define SLEEK_EEB (-4)         /* error bound too large */
+#define SLEEK_ECAPACITY (-5)   /* decoded size exceeds the output capacity */
+#define SLEEK_EWORKSPACE (-6)  /* workspace too small */
+#define SLEEK_EALIGN (-7)      /* pointer not aligned as required */
+
+#define SLEEK_CHUNK_BYTES 16384
+
+/* ---- helpers ---- */
+
+/* Bytes a stream of insize input bytes may need; same value as the
+ * reference's sleek_upper_bound_size (sleek_common_gpu.h:79-83). */
+long long sleek_gpu_upper_bound_size(long long insize);
+
+/* Platform checks of sleek_system_checks (sleek_common_gpu.h:86-105): 0 if ok. */
+int sleek_gpu_system_checks(void);
+
+/* Suggested persistent grid size for the current device (the role of
+ * sleek_thread_blocks, sleek_common_gpu.h:108-119).  Output bytes never
+ * depend on the grid size.  Returns a positive count, or a negative status. */
+int sleek_gpu_thread_blocks(void);
+
+/* Error-bound parameters of an absolute bound (sleek_lossy_gpu.h:56-71).
+ * Returns SLEEK_EEB where the reference exits the process. */
+int sleek_gpu_error_bound_f32(float eb, int* eb_e, int* thr_e, unsigned int* offs_e);
+int sleek_gpu_error_bound_f64(double eb, int* eb_e, int* thr_e, unsigned long long* offs_e);
+
+/* Text of a status code, and the last HIP error message of this thread. */
+const char* sleek_gpu_strerror(int status);
+const char* sleek_gpu_last_error(void);
+
+/* ---- synchronous codec calls (library-owned scratch, null stream) ----
+ * Counterparts of sleek_compress_gpu / sleek_decompress_gpu:
+ * lossless sleek_lossless_gpu.h:47-100, lossy sleek_lossy_gpu.h:74-127.
+ * d_input / d_decoded must be 16-byte aligned, d_encoded 8-byte aligned;
+ * d_encoded holds at least sleek_gpu_upper_bound_size(insize) bytes.
+ * *d_encsize / *d_decsize are device int64 results.  blocks <= 0 picks the
+ * grid; any positive value is a persistent grid size (bytes do not change).
+ * On return the results are complete; a malformed stream gives
+ * SLEEK_ECORRUPT (the reference traps the device there). */
+int sleek_gpu_compress_f32(int blocks, const float* d_input, long long num_elements,
+                           unsigned char* d_encoded, long long* d_encsize);
+int sleek_gpu_decompress_f32(int blocks, const unsigned char* d_encoded, float* d_decoded,
+                             long long* d_decsize);
+int sleek_gpu_compress_f64(int blocks, const double* d_input, long long num_elements,
+                           unsigned char* d_encoded, long long* d_encsize);
+int sleek_gpu_decompress_f64(int blocks, const unsigned char* d_encoded, double* d_decoded,
+                             long long* d_decsize);
+int sleek_gpu_compress_lossy_f32(int blocks, int eb_e, int thr_e, unsigned int offs_e,
+                                 const float* d_input, long long num_elements,
+                                 unsigned char* d_encoded, long long* d_encsize);
+int sleek_gpu_decompress_lossy_f32(int blocks, int eb_e, int thr_e, unsigned int offs_e,
+                                   const unsigned char* d_encoded, float* d_decoded,
+                                   long long* d_decsize);
+int sleek_gpu_compress_lossy_f64(int blocks, int eb_e, int thr_e, unsigned long long offs_e,
+                                 const double* d_input, long long num_elements,
+                                 unsigned char* d_encoded, long long* d_encsize);
+int sleek_gpu_decompress_lossy_f64(int blocks, int eb_e, int thr_e, unsigned long long offs_e,
+                                   const unsigned char* d_encoded, double* d_decoded,
+                                   long long* d_decsize);
+
+/* ---- stream-ordered calls with a caller-owned workspace ----
+ * No reference counterpart: the reference allocates and frees its scratch
+ * inside every call (sleek_lossless_gpu.h:52-59), which serialises the
+ * device.  These enqueue on `stream` and return immediately; the workspace
+ * must not be shared by calls that may overlap.
+ * elem_bytes: 4 (float32) or 8 (float64).  lossy: 0 or 1.
+ * enclen / capacity: stream bytes available and decoded-buffer bytes, or -1
+ * if unknown (capacity -1 needs a workspace sized for the stream header's
+ * length).  Device-detected stream errors are read back, after the stream
+ * has drained, by sleek_gpu_workspace_status. */
+long long sleek_gpu_workspace_size(long long max_insize);
+int sleek_gpu_compress_async(int elem_bytes, int lossy, int eb_e, int thr_e, unsigned long long offs_e,
+                             const void* d_input, long long insize, unsigned char* d_encoded,
+                             long long* d_encsize, void* d_workspace, long long workspace_bytes,
+                             int blocks, void* stream);
+int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned long long offs_e,
+                               const unsigned char* d_encoded, long long enclen, void* d_decoded,
+                               long long capacity, long long* d_decsize, void* d_workspace,
+                               long long workspace_bytes, int blocks, void* stream);
+/* Synchronises `stream`, then returns SLEEK_OK, SLEEK_ECORRUPT or
+ * SLEEK_ECAPACITY from the workspace's status word. */
+int sleek_gpu_workspace_status(const void* d_workspace, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

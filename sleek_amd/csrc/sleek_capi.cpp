@@ -1,0 +1,322 @@
+// C ABI of the SLEEK GPU codec (declared in include/sleek_gpu.h).
+//
+// Argument checks, workspace management and kernel launches.  The
+// synchronous entry points mirror the reference host API
+// (sleek_lossless_gpu.h:47-100, sleek_lossy_gpu.h:56-127) with status codes
+// in place of exit() / device traps; the *_async entry points are
+// stream-ordered and take a caller-owned workspace.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <string.h>
+
+#include <mutex>
+#include <vector>
+
+#include "../../include/sleek_gpu.h"
+#include "sleek_internal.h"
+
+namespace {
+
+thread_local char g_last_error[256] = "";
+
+int hip_fail(hipError_t e, const char* what)
+{
+  snprintf(g_last_error, sizeof(g_last_error), "%s: %s", what, hipGetErrorString(e));
+  return SLEEK_EHIP;
+}
+
+long long chunks_of(long long bytes) { return (bytes + SLEEK_CHUNK_BYTES - 1) / SLEEK_CHUNK_BYTES; }
+
+long long ws_bytes_for(long long insize) { return (8 + chunks_of(insize)) * 8; }
+
+// Per-device scratch of the synchronous entry points (grow-only).
+struct DeviceScratch {
+  void* ptr = nullptr;
+  long long bytes = 0;
+};
+std::mutex g_mu;
+std::vector<DeviceScratch> g_scratch;
+
+int scratch_for(long long need, void** out)
+{
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1);
+  DeviceScratch& s = g_scratch[dev];
+  if (s.bytes < need) {
+    if (s.ptr) {
+      e = hipFree(s.ptr);
+      if (e != hipSuccess) return hip_fail(e, "hipFree");
+      s.ptr = nullptr;
+      s.bytes = 0;
+    }
+    const long long sz = need < (1 << 20) ? (1 << 20) : need;
+    e = hipMalloc(&s.ptr, (size_t)sz);
+    if (e != hipSuccess) return hip_fail(e, "hipMalloc");
+    s.bytes = sz;
+  }
+  *out = s.ptr;
+  return SLEEK_OK;
+}
+
+int grid_for(int blocks, int elem_bytes, bool lossy, bool decode, long long nchunks)
+{
+  long long g = blocks > 0 ? blocks : sleek::resident_grid(elem_bytes, lossy, decode);
+  if (g <= 0) g = 1024;
+  if (nchunks >= 0 && g > nchunks) g = nchunks > 0 ? nchunks : 1;
+  if (g > (1 << 20)) g = 1 << 20;
+  return (int)g;
+}
+
+bool aligned(const void* p, uintptr_t a) { return ((uintptr_t)p & (a - 1)) == 0; }
+
+int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long long offs, const void* d_input,
+                  long long insize, unsigned char* d_encoded, long long* d_encsize, void* ws, long long ws_bytes,
+                  int blocks, hipStream_t stream)
+{
+  if ((elem_bytes != 4 && elem_bytes != 8) || insize < 0 || insize % elem_bytes != 0) return SLEEK_EINVAL;
+  if (!d_encoded || !d_encsize || !ws || (insize > 0 && !d_input)) return SLEEK_EINVAL;
+  if (!aligned(d_input, 16) || !aligned(d_encoded, 8) || !aligned(d_encsize, 8) || !aligned(ws, 8)) return SLEEK_EALIGN;
+  const long long need = ws_bytes_for(insize);
+  if (ws_bytes < need) return SLEEK_EWORKSPACE;
+  hipError_t e = hipMemsetAsync(ws, 0, (size_t)need, stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  sleek::EncodeArgs a;
+  a.in = (const uint8_t*)d_input;
+  a.insize = insize;
+  a.out = d_encoded;
+  a.outsize = d_encsize;
+  a.ws = (uint64_t*)ws;
+  a.eb_e = eb_e;
+  a.thr_e = thr_e;
+  a.offs = offs;
+  const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
+  e = sleek::encode_launch(elem_bytes, lossy, a, grid, stream);
+  if (e != hipSuccess) return hip_fail(e, "encode launch");
+  return SLEEK_OK;
+}
+
+int decompress_impl(int elem_bytes, bool lossy, int thr_e, unsigned long long offs, const unsigned char* d_encoded,
+                    long long enclen, void* d_decoded, long long capacity, long long* d_decsize, void* ws,
+                    long long ws_bytes, int blocks, hipStream_t stream)
+{
+  if ((elem_bytes != 4 && elem_bytes != 8) || !d_encoded || !d_decsize || !ws) return SLEEK_EINVAL;
+  if (enclen >= 0 && enclen < 8) return SLEEK_ECORRUPT;
+  if (!aligned(d_decoded, 16) || !aligned(d_encoded, 8) || !aligned(d_decsize, 8) || !aligned(ws, 8)) return SLEEK_EALIGN;
+  if (capacity >= 0) {
+    if (ws_bytes < ws_bytes_for(capacity)) return SLEEK_EWORKSPACE;
+  }
+  // without a capacity the workspace bounds the chunk count
+  const long long ws_chunks = ws_bytes / 8 - 8;
+  if (ws_chunks < 0) return SLEEK_EWORKSPACE;
+  const long long cap = capacity >= 0 ? capacity : ws_chunks * SLEEK_CHUNK_BYTES;
+  hipError_t e = hipMemsetAsync(ws, 0, (size_t)ws_bytes_for(cap), stream);
+  if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  sleek::DecodeArgs a;
+  a.in = d_encoded;
+  a.inlen = enclen;
+  a.out = (uint8_t*)d_decoded;
+  a.cap = cap;
+  a.decsize = d_decsize;
+  a.ws = (uint64_t*)ws;
+  a.thr_e = thr_e;
+  a.offs = offs;
+  const int grid = grid_for(blocks, elem_bytes, lossy, true, chunks_of(cap));
+  e = sleek::decode_launch(elem_bytes, lossy, a, grid, stream);
+  if (e != hipSuccess) return hip_fail(e, "decode launch");
+  return SLEEK_OK;
+}
+
+int read_status(const void* ws, hipStream_t stream)
+{
+  hipError_t e = hipStreamSynchronize(stream);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  unsigned long long st = 0;
+  e = hipMemcpy(&st, (const uint64_t*)ws + 1, 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy status");
+  if (st & 2) return SLEEK_ECAPACITY;
+  if (st & 1) return SLEEK_ECORRUPT;
+  return SLEEK_OK;
+}
+
+int compress_sync(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long long offs, const void* d_input,
+                  long long num_elements, unsigned char* d_encoded, long long* d_encsize, int blocks)
+{
+  if (num_elements < 0) return SLEEK_EINVAL;
+  std::lock_guard<std::mutex> lk(g_mu);
+  const long long insize = num_elements * elem_bytes;
+  void* ws = nullptr;
+  int rc = scratch_for(ws_bytes_for(insize), &ws);
+  if (rc) return rc;
+  rc = compress_impl(elem_bytes, lossy, eb_e, thr_e, offs, d_input, insize, d_encoded, d_encsize, ws,
+                     ws_bytes_for(insize), blocks, 0);
+  if (rc) return rc;
+  return read_status(ws, 0);
+}
+
+int decompress_sync(int elem_bytes, bool lossy, int thr_e, unsigned long long offs, const unsigned char* d_encoded,
+                    void* d_decoded, long long* d_decsize, int blocks)
+{
+  if (!d_encoded) return SLEEK_EINVAL;
+  if (!aligned(d_encoded, 8)) return SLEEK_EALIGN;
+  std::lock_guard<std::mutex> lk(g_mu);
+  // the decoded length sizes the scratch (the reference's callers read it the same way)
+  long long hdr = 0;
+  hipError_t e = hipMemcpy(&hdr, d_encoded, 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy header");
+  if (hdr < 0) return SLEEK_ECORRUPT;
+  void* ws = nullptr;
+  int rc = scratch_for(ws_bytes_for(hdr), &ws);
+  if (rc) return rc;
+  rc = decompress_impl(elem_bytes, lossy, thr_e, offs, d_encoded, -1, d_decoded, -1, d_decsize, ws,
+                       ws_bytes_for(hdr), blocks, 0);
+  if (rc) return rc;
+  return read_status(ws, 0);
+}
+
+}  // namespace
+
+extern "C" {
+
+long long sleek_gpu_upper_bound_size(long long insize)
+{
+  const long long chunks = chunks_of(insize);
+  return 3 * 4 + chunks * 2 + chunks * SLEEK_CHUNK_BYTES;
+}
+
+int sleek_gpu_system_checks(void)
+{
+  const int one = 1;
+  if (*(const char*)&one != 1) return SLEEK_EINVAL;
+  if (sizeof(long long) != 8 || sizeof(int) != 4 || sizeof(short) != 2) return SLEEK_EINVAL;
+  return SLEEK_OK;
+}
+
+int sleek_gpu_thread_blocks(void)
+{
+  const int g = sleek::resident_grid(4, false, false);
+  return g > 0 ? g : SLEEK_EHIP;
+}
+
+int sleek_gpu_error_bound_f32(float eb, int* eb_e, int* thr_e, unsigned int* offs_e)
+{
+  if (!eb_e || !thr_e || !offs_e) return SLEEK_EINVAL;
+  uint32_t bits;
+  memcpy(&bits, &eb, 4);
+  const int e = (int)((bits >> 23) & 0xffu);
+  const int t = e + 24;
+  if (t >= 255) return SLEEK_EEB;
+  *eb_e = e;
+  *thr_e = t;
+  *offs_e = ((uint32_t)t << 23) - (1u << 23);
+  return SLEEK_OK;
+}
+
+int sleek_gpu_error_bound_f64(double eb, int* eb_e, int* thr_e, unsigned long long* offs_e)
+{
+  if (!eb_e || !thr_e || !offs_e) return SLEEK_EINVAL;
+  uint64_t bits;
+  memcpy(&bits, &eb, 8);
+  const int e = (int)((bits >> 52) & 0x7ffu);
+  const int t = e + 53;
+  if (t >= 2047) return SLEEK_EEB;
+  *eb_e = e;
+  *thr_e = t;
+  *offs_e = ((unsigned long long)t << 52) - (1ull << 52);
+  return SLEEK_OK;
+}
+
+const char* sleek_gpu_strerror(int status)
+{
+  switch (status) {
+    case SLEEK_OK: return "ok";
+    case SLEEK_EINVAL: return "invalid argument";
+    case SLEEK_EHIP: return "HIP runtime error";
+    case SLEEK_ECORRUPT: return "malformed compressed stream";
+    case SLEEK_EEB: return "error bound is too large";
+    case SLEEK_ECAPACITY: return "decoded size exceeds the output capacity";
+    case SLEEK_EWORKSPACE: return "workspace too small";
+    case SLEEK_EALIGN: return "pointer not aligned (16 B for arrays, 8 B for streams)";
+    default: return "unknown status";
+  }
+}
+
+const char* sleek_gpu_last_error(void) { return g_last_error; }
+
+int sleek_gpu_compress_f32(int blocks, const float* d_input, long long n, unsigned char* d_encoded, long long* d_encsize)
+{
+  return compress_sync(4, false, 0, 0, 0, d_input, n, d_encoded, d_encsize, blocks);
+}
+
+int sleek_gpu_decompress_f32(int blocks, const unsigned char* d_encoded, float* d_decoded, long long* d_decsize)
+{
+  return decompress_sync(4, false, 0, 0, d_encoded, d_decoded, d_decsize, blocks);
+}
+
+int sleek_gpu_compress_f64(int blocks, const double* d_input, long long n, unsigned char* d_encoded, long long* d_encsize)
+{
+  return compress_sync(8, false, 0, 0, 0, d_input, n, d_encoded, d_encsize, blocks);
+}
+
+int sleek_gpu_decompress_f64(int blocks, const unsigned char* d_encoded, double* d_decoded, long long* d_decsize)
+{
+  return decompress_sync(8, false, 0, 0, d_encoded, d_decoded, d_decsize, blocks);
+}
+
+int sleek_gpu_compress_lossy_f32(int blocks, int eb_e, int thr_e, unsigned int offs_e, const float* d_input,
+                                 long long n, unsigned char* d_encoded, long long* d_encsize)
+{
+  return compress_sync(4, true, eb_e, thr_e, offs_e, d_input, n, d_encoded, d_encsize, blocks);
+}
+
+int sleek_gpu_decompress_lossy_f32(int blocks, int eb_e, int thr_e, unsigned int offs_e,
+                                   const unsigned char* d_encoded, float* d_decoded, long long* d_decsize)
+{
+  (void)eb_e;
+  return decompress_sync(4, true, thr_e, offs_e, d_encoded, d_decoded, d_decsize, blocks);
+}
+
+int sleek_gpu_compress_lossy_f64(int blocks, int eb_e, int thr_e, unsigned long long offs_e, const double* d_input,
+                                 long long n, unsigned char* d_encoded, long long* d_encsize)
+{
+  return compress_sync(8, true, eb_e, thr_e, offs_e, d_input, n, d_encoded, d_encsize, blocks);
+}
+
+int sleek_gpu_decompress_lossy_f64(int blocks, int eb_e, int thr_e, unsigned long long offs_e,
+                                   const unsigned char* d_encoded, double* d_decoded, long long* d_decsize)
+{
+  (void)eb_e;
+  return decompress_sync(8, true, thr_e, offs_e, d_encoded, d_decoded, d_decsize, blocks);
+}
+
+long long sleek_gpu_workspace_size(long long max_insize)
+{
+  if (max_insize < 0) return SLEEK_EINVAL;
+  return ws_bytes_for(max_insize);
+}
+
+int sleek_gpu_compress_async(int elem_bytes, int lossy, int eb_e, int thr_e, unsigned long long offs_e,
+                             const void* d_input, long long insize, unsigned char* d_encoded, long long* d_encsize,
+                             void* d_workspace, long long workspace_bytes, int blocks, void* stream)
+{
+  return compress_impl(elem_bytes, lossy != 0, eb_e, thr_e, offs_e, d_input, insize, d_encoded, d_encsize,
+                       d_workspace, workspace_bytes, blocks, (hipStream_t)stream);
+}
+
+int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned long long offs_e,
+                               const unsigned char* d_encoded, long long enclen, void* d_decoded, long long capacity,
+                               long long* d_decsize, void* d_workspace, long long workspace_bytes, int blocks,
+                               void* stream)
+{
+  return decompress_impl(elem_bytes, lossy != 0, thr_e, offs_e, d_encoded, enclen, d_decoded, capacity, d_decsize,
+                         d_workspace, workspace_bytes, blocks, (hipStream_t)stream);
+}
+
+int sleek_gpu_workspace_status(const void* d_workspace, void* stream)
+{
+  if (!d_workspace) return SLEEK_EINVAL;
+  return read_status(d_workspace, (hipStream_t)stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,317 @@
+// Device-side building blocks of the SLEEK codec for gfx950 (CDNA4, wave64).
+//
+// Format constants, the per-value integer transforms, wave primitives and the
+// LDS <-> global byte movers shared by the plain and the batched kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sleek {
+
+// ---- format constants (sleek_common_gpu.h:54, :317-318 of the reference) ----
+constexpr int kChunk = 16384;            // bytes of input per chunk
+constexpr int kSub = 32;                 // subchunks per chunk
+constexpr int kSubBytes = kChunk / kSub; // 512 bytes per subchunk
+
+// ---- launch geometry ----
+constexpr int kThreads = 256;                       // 4 waves of 64
+constexpr int kRounds = kChunk / (kThreads * 16);   // 16-byte slices per thread and chunk
+constexpr int kStageBytes = kChunk + 64;            // LDS staging buffer (payload + realignment slack)
+
+// ---- scan state words ----
+// Workspace layout (uint64 words): [0] ticket, [1] status bits, [8..) one
+// look-back word per chunk.  A look-back word holds the value in bits 61..0
+// and its kind in bits 63..62: 0 = not published, 1 = aggregate (the
+// chunk's own compressed size), 2 = inclusive (offset just past the chunk).
+constexpr int kWsTicket = 0;
+constexpr int kWsStatus = 1;
+constexpr int kWsFlags = 8;
+constexpr uint64_t kAgg = 1ull << 62;
+constexpr uint64_t kInc = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+
+// status bits reported by the kernels
+constexpr uint64_t kStCorrupt = 1;   // malformed stream
+constexpr uint64_t kStCapacity = 2;  // decoded size exceeds the caller's buffer
+
+// ---- element traits ----
+template <typename W> struct Word;
+template <> struct Word<uint32_t> {
+  static constexpr int TB = 32, EXPB = 8, MANB = 23;
+  static constexpr int PerSlice = 4;    // values per 16-byte slice
+  static constexpr int PerSub = 128;    // values per subchunk
+};
+template <> struct Word<uint64_t> {
+  static constexpr int TB = 64, EXPB = 11, MANB = 52;
+  static constexpr int PerSlice = 2;
+  static constexpr int PerSub = 64;
+};
+
+__device__ __forceinline__ int clz_w(uint32_t x) { return __clz(x); }
+__device__ __forceinline__ int clz_w(uint64_t x) { return __clzll(x); }
+
+// ---- value transforms.  The steps are fixed by the stream format ----
+
+// Lossless code (reference sleek_common_cpu.h:152-169): rotate the sign into
+// bit 0, re-centre a non-zero exponent around 1.0, zigzag-fold.
+template <typename W>
+__device__ __forceinline__ W lossless_code(W v)
+{
+  constexpr int TB = Word<W>::TB;
+  constexpr W kTop = (W)((((W)1 << Word<W>::EXPB) - 1)) << (Word<W>::MANB + 1);
+  constexpr W kHalf = (W)1 << (TB - 1);
+  constexpr W kOne = (W)1 << (Word<W>::MANB + 1);
+  v = (v << 1) | (v >> (TB - 1));
+  if (v & kTop) {
+    v -= kHalf;
+    if (!(v & kTop) || (v & kHalf)) v -= kOne;
+  }
+  const W fold = (W)0 - (v >> (TB - 1));
+  return (v << 1) ^ fold;
+}
+
+// Inverse of lossless_code (reference sleek_common_cpu.h:317-335).
+template <typename W>
+__device__ __forceinline__ W lossless_value(W v)
+{
+  constexpr int TB = Word<W>::TB;
+  constexpr W kTop = (W)((((W)1 << Word<W>::EXPB) - 1)) << (Word<W>::MANB + 1);
+  constexpr W kHalf = (W)1 << (TB - 1);
+  constexpr W kOne = (W)1 << (Word<W>::MANB + 1);
+  v = (v >> 1) ^ ((W)0 - (v & 1));
+  if (v & kTop) {
+    if (v & kHalf) v += kOne;
+    v += kHalf;
+  }
+  return (v << (TB - 1)) | (v >> 1);
+}
+
+// Error-bounded code (reference sleek_common_cpu.h:393-419): magnitudes at or
+// above 2^thr keep their bits (minus an offset), magnitudes in [2^eb, 2^thr)
+// are rounded to bins of width 2^(eb+1), smaller ones become 0; the sign is
+// folded into bit 0 with both zeros mapping to 0.  Unsigned arithmetic equals
+// the reference's wrapping signed arithmetic modulo 2^TB.
+template <typename W>
+__device__ __forceinline__ W lossy_code(W val, int eb_e, int thr_e, W offs)
+{
+  constexpr int EM = Word<W>::EXPB + Word<W>::MANB;
+  constexpr int M = Word<W>::MANB;
+  const W mag = val & (((W)1 << EM) - 1);
+  const int e = (int)(mag >> M);
+  W enc = 0;
+  if (e >= thr_e) {
+    enc = mag - offs;
+  } else if (e >= eb_e) {
+    const int sh = thr_e - e;
+    const W mant = (val & (((W)1 << M) - 1)) | ((W)1 << M);
+    enc = (mant + ((W)1 << (sh - 1))) >> sh;
+  }
+  enc = (enc << 1) | ((~val) >> EM);
+  return enc - (enc != 0 ? (W)1 : (W)0);
+}
+
+// Inverse of lossy_code (reference sleek_common_cpu.h:559-591).
+template <typename W>
+__device__ __forceinline__ W lossy_value(W enc, int thr_e, W offs)
+{
+  constexpr int TB = Word<W>::TB;
+  constexpr int M = Word<W>::MANB;
+  constexpr int EM = Word<W>::EXPB + Word<W>::MANB;
+  if (enc == 0) return 0;
+  const W mag = (W)(enc + 1) >> 1;
+  W dec = 0;
+  if (mag >= ((W)1 << M)) {
+    dec = mag + offs;
+  } else {
+    const int sh = clz_w(mag) - (TB - 1 - M);
+    dec = ((mag << sh) & (((W)1 << M) - 1)) | ((W)(thr_e - sh) << M);
+  }
+  return dec | (enc << EM);
+}
+
+template <typename W, bool LOSSY>
+__device__ __forceinline__ W to_code(W v, int eb_e, int thr_e, W offs)
+{
+  if constexpr (LOSSY) return lossy_code<W>(v, eb_e, thr_e, offs);
+  else return lossless_code<W>(v);
+}
+
+template <typename W, bool LOSSY>
+__device__ __forceinline__ W from_code(W c, int thr_e, W offs)
+{
+  if constexpr (LOSSY) return lossy_value<W>(c, thr_e, offs);
+  else return lossless_value<W>(c);
+}
+
+// ---- 16-byte slice <-> values ----
+__device__ __forceinline__ void slice_values(const uint4& q, uint32_t v[4]) { v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w; }
+__device__ __forceinline__ void slice_values(const uint4& q, uint64_t v[2])
+{
+  v[0] = (uint64_t)q.x | ((uint64_t)q.y << 32);
+  v[1] = (uint64_t)q.z | ((uint64_t)q.w << 32);
+}
+__device__ __forceinline__ uint4 values_slice(const uint32_t v[4]) { return make_uint4(v[0], v[1], v[2], v[3]); }
+__device__ __forceinline__ uint4 values_slice(const uint64_t v[2])
+{
+  return make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
+}
+
+__device__ __forceinline__ int bit_length(uint32_t x) { return 32 - __clz(x); }
+__device__ __forceinline__ int bit_length(uint64_t x) { return 64 - __clzll(x); }
+
+// max over the 32 lanes of a half-wave (ds_swizzle bit mode, xor 1..16)
+__device__ __forceinline__ int half_wave_max(int v)
+{
+  v = max(v, __builtin_amdgcn_ds_swizzle(v, 0x041F));
+  v = max(v, __builtin_amdgcn_ds_swizzle(v, 0x081F));
+  v = max(v, __builtin_amdgcn_ds_swizzle(v, 0x101F));
+  v = max(v, __builtin_amdgcn_ds_swizzle(v, 0x201F));
+  v = max(v, __builtin_amdgcn_ds_swizzle(v, 0x401F));
+  return v;
+}
+
+// sum over the 64 lanes of a wave
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v)
+{
+#pragma unroll
+  for (int m = 1; m < 64; m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+// Broadcast lane 0's value: makes block-uniform values (read from LDS or
+// global memory by every lane) scalar, so branches on them are scalar
+// branches and loops containing barriers keep a single, uniform latch.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
+__device__ __forceinline__ long long uni(long long v)
+{
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((unsigned long long)v >> 32));
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+
+// Any byte of x above n (n <= 127), without carries between bytes.
+__device__ __forceinline__ bool any_byte_above(uint32_t x, uint32_t n)
+{
+  return (((x + 0x01010101u * (127u - n)) | x) & 0x80808080u) != 0u;
+}
+
+// Sum of the four bytes of x.
+__device__ __forceinline__ uint32_t byte_sum(uint32_t x) { return __builtin_amdgcn_sad_u8(x, 0u, 0u); }
+
+// Bit offset (relative to the first packed bit) of subchunk s, given the 32
+// per-subchunk bit lengths as 8 little-endian byte-packed dwords.
+template <typename W>
+__device__ __forceinline__ int sub_bit_offset(const uint32_t lg[8], int s)
+{
+  uint32_t acc = 0;
+  const int full = s >> 2;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    if (i < full) acc += byte_sum(lg[i]);
+  }
+  const int part = s & 3;
+  if (part) {
+    const uint32_t w = lg[full] & ((1u << (8 * part)) - 1u);
+    acc += byte_sum(w);
+  }
+  return (int)acc * Word<W>::PerSub;
+}
+
+__device__ __forceinline__ int sub_length(const uint32_t lg[8], int s) { return (int)((lg[s >> 2] >> (8 * (s & 3))) & 0xffu); }
+
+// ---- scan state ----
+__device__ __forceinline__ uint64_t flag_load(const uint64_t* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void flag_store(uint64_t* p, uint64_t v)
+{
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long take_ticket(uint64_t* ws)
+{
+  return __hip_atomic_fetch_add((unsigned long long*)&ws[kWsTicket], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void raise_status(uint64_t* ws, uint64_t bits)
+{
+  __hip_atomic_fetch_or((unsigned long long*)&ws[kWsStatus], (unsigned long long)bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Decoupled look-back over the chunk-order prefix of compressed sizes.
+// Called by one whole wave; returns the exclusive prefix of chunk c (the sum
+// of the sizes of chunks 0..c-1) in every lane.  Each lane watches one
+// predecessor of a 64-wide window that slides back until it meets an
+// inclusive word.  With DECODE the sizes are already known from the size
+// table, so an unpublished predecessor contributes its table entry and the
+// wave never waits.
+template <bool DECODE>
+__device__ __forceinline__ uint64_t look_back(const uint64_t* flags, long long c, const uint16_t* sizes)
+{
+  const int lane = threadIdx.x & 63;
+  uint64_t excl = 0;
+  long long end = c;
+  while (end > 0) {
+    const long long j = end - 1 - lane;
+    while (true) {
+      uint64_t f = kInc;   // before chunk 0: inclusive 0
+      if (j >= 0) {
+        f = flag_load(flags + j);
+        if (DECODE && (f >> 62) == 0) f = kAgg | (uint64_t)sizes[j];
+      }
+      const uint64_t inc = __ballot((f >> 62) == 2);
+      const uint64_t rdy = __ballot((f >> 62) != 0);
+      if (inc) {
+        const int p = __ffsll((unsigned long long)inc) - 1;
+        const uint64_t need = (p == 63) ? ~0ull : ((2ull << p) - 1);
+        if ((rdy & need) == need) {
+          excl += wave_sum(lane <= p ? (f & kValMask) : 0);
+          return excl;
+        }
+      } else if (rdy == ~0ull) {
+        excl += wave_sum(f & kValMask);
+        end -= 64;
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+  }
+  return excl;
+}
+
+// 16 bytes starting at an arbitrary byte offset of a dword-aligned LDS buffer
+// (reads up to 4 bytes past the slice).
+__device__ __forceinline__ uint4 lds_read16(const uint32_t* lds, int byte_off)
+{
+  const int q = byte_off >> 2;
+  const uint32_t sh = (uint32_t)(byte_off & 3) * 8u;
+  const uint32_t x0 = lds[q], x1 = lds[q + 1], x2 = lds[q + 2], x3 = lds[q + 3], x4 = lds[q + 4];
+  return make_uint4(__builtin_amdgcn_alignbit(x1, x0, sh), __builtin_amdgcn_alignbit(x2, x1, sh),
+                    __builtin_amdgcn_alignbit(x3, x2, sh), __builtin_amdgcn_alignbit(x4, x3, sh));
+}
+
+// Copy len bytes from LDS (byte 0 of lds) to global memory at an arbitrary
+// byte address: bytes up to the first 16-byte boundary and after the last
+// one are stored singly (neighbouring chunks own the rest of those lines),
+// the body with 16-byte stores.  Cooperative over the workgroup.
+__device__ __forceinline__ void store_unaligned(uint8_t* dst, const uint32_t* lds, int len)
+{
+  const int t = threadIdx.x;
+  const int mis = (int)((uintptr_t)dst & 15);
+  const int head = min(len, (16 - mis) & 15);
+  const int nfull = (len - head) >> 4;
+  const int tail = len - head - (nfull << 4);
+  const uint8_t* lb = (const uint8_t*)lds;
+  if (t < head) dst[t] = lb[t];
+  if (t >= 64 && t < 64 + tail) {
+    const int i = head + (nfull << 4) + (t - 64);
+    dst[i] = lb[i];
+  }
+  for (int j = t; j < nfull; j += kThreads) {
+    const int p = head + (j << 4);
+    *(uint4*)(dst + p) = lds_read16(lds, p);
+  }
+}
+
+}  // namespace sleek

@@ -1,0 +1,34 @@
+// Host-visible declarations shared by the kernels and the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sleek {
+
+struct EncodeArgs {
+  const uint8_t* in;      // input array (16-byte aligned)
+  long long insize;       // bytes
+  uint8_t* out;           // stream (8-byte aligned, >= upper-bound size)
+  long long* outsize;     // device: stream length in bytes
+  uint64_t* ws;           // zeroed workspace: ticket, status, look-back words
+  int eb_e, thr_e;        // lossy parameters (ignored for lossless)
+  unsigned long long offs;
+};
+
+struct DecodeArgs {
+  const uint8_t* in;      // stream (8-byte aligned)
+  long long inlen;        // stream bytes available, or -1 if unknown
+  uint8_t* out;           // decoded array (16-byte aligned)
+  long long cap;          // capacity of out in bytes, or -1 if unknown
+  long long* decsize;     // device: decoded length in bytes
+  uint64_t* ws;           // zeroed workspace
+  int thr_e;
+  unsigned long long offs;
+};
+
+hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, hipStream_t stream);
+hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream);
+int resident_grid(int elem_bytes, bool lossy, bool decode);
+
+}  // namespace sleek

@@ -1,0 +1,348 @@
+// SLEEK encode / decode kernels for gfx950.
+//
+// One workgroup (4 wave64s) codes one 16 KiB chunk at a time and loops over
+// chunk tickets until the array is done (persistent grid).  Per chunk:
+//   encode: 16-byte coalesced loads (4 slices per thread) -> per-value code in
+//           registers -> per-subchunk bit length by a half-wave max ->
+//           compressed/raw decision -> publish the size and run a decoupled
+//           look-back (wave 0) while the other waves OR-pack the codes into
+//           an LDS staging buffer -> realigned 16-byte stores to the stream.
+//   decode: look-back over the size table (never waits: unpublished
+//           predecessors contribute their table entry) -> aligned 16-byte
+//           loads of the payload's cover into LDS -> per-lane bit extraction
+//           -> inverse transform -> 16-byte coalesced stores.
+// Reference counterparts: d_encode / d_decode, sleek_common_gpu.h:474-768
+// (lossless) and :989-1279 (lossy).
+#include "sleek_device.h"
+#include "sleek_internal.h"
+
+namespace sleek {
+
+template <typename W>
+__device__ __forceinline__ void or_code(uint32_t* st, int b, W v);
+
+template <>
+__device__ __forceinline__ void or_code<uint32_t>(uint32_t* st, int b, uint32_t v)
+{
+  const int d = b >> 5;
+  const uint32_t sh = (uint32_t)b & 31u;
+  atomicOr(&st[d], v << sh);
+  if (sh) {
+    const uint32_t hi = v >> (32u - sh);
+    if (hi) atomicOr(&st[d + 1], hi);
+  }
+}
+
+template <>
+__device__ __forceinline__ void or_code<uint64_t>(uint32_t* st, int b, uint64_t v)
+{
+  const int d = b >> 5;
+  const uint32_t sh = (uint32_t)b & 31u;
+  const uint64_t lo = v << sh;
+  const uint32_t w0 = (uint32_t)lo, w1 = (uint32_t)(lo >> 32);
+  const uint32_t w2 = sh ? (uint32_t)(v >> (64u - sh)) : 0u;
+  if (w0) atomicOr(&st[d], w0);
+  if (w1) atomicOr(&st[d + 1], w1);
+  if (w2) atomicOr(&st[d + 2], w2);
+}
+
+__device__ __forceinline__ uint32_t extract_code(const uint32_t* st, int b, int ln, uint32_t*)
+{
+  const int d = b >> 5;
+  const uint32_t v = __builtin_amdgcn_alignbit(st[d + 1], st[d], (uint32_t)b & 31u);
+  return ln >= 32 ? v : (v & ((1u << ln) - 1u));
+}
+
+__device__ __forceinline__ uint64_t extract_code(const uint32_t* st, int b, int ln, uint64_t*)
+{
+  const int d = b >> 5;
+  const uint32_t sh = (uint32_t)b & 31u;
+  const uint32_t w0 = st[d], w1 = st[d + 1], w2 = st[d + 2];
+  const uint64_t v = (uint64_t)__builtin_amdgcn_alignbit(w1, w0, sh) | ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, sh) << 32);
+  return ln >= 64 ? v : (v & ((1ull << ln) - 1ull));
+}
+
+// Load slice r of a chunk of osize bytes (zero beyond osize).
+__device__ __forceinline__ uint4 load_slice(const uint8_t* src, int off, int osize)
+{
+  if (off + 16 <= osize) return *(const uint4*)(src + off);
+  uint32_t d[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (off + 4 * i + 4 <= osize) d[i] = ((const uint32_t*)(src + off))[i];
+  return make_uint4(d[0], d[1], d[2], d[3]);
+}
+
+__device__ __forceinline__ void store_slice(uint8_t* dst, int off, int osize, uint4 q)
+{
+  if (off + 16 <= osize) {
+    *(uint4*)(dst + off) = q;
+    return;
+  }
+  const uint32_t d[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    if (off + 4 * i + 4 <= osize) ((uint32_t*)(dst + off))[i] = d[i];
+}
+
+template <typename W, bool LOSSY>
+__global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a)
+{
+  constexpr int PS = Word<W>::PerSlice;
+  __shared__ uint32_t stage[kStageBytes / 4];
+  __shared__ uint32_t lgs[8];
+  __shared__ long long s_chunk;
+  __shared__ unsigned long long s_excl;
+
+  const int t = threadIdx.x;
+  const int lane32 = t & 31;
+  const int half = t >> 5;
+  const long long nchunks = (a.insize + kChunk - 1) / kChunk;
+  const long long data_base = 8 + 2 * nchunks;
+  uint16_t* const sizes = (uint16_t*)(a.out + 8);
+  uint64_t* const flags = a.ws + kWsFlags;
+  const W offs = (W)a.offs;
+
+  if (nchunks == 0) {
+    if (blockIdx.x == 0 && t == 0) {
+      *(long long*)a.out = 0;
+      *a.outsize = 8;
+    }
+    return;
+  }
+
+  for (;;) {
+    if (t == 0) s_chunk = (long long)take_ticket(a.ws);
+    for (int i = t; i < kChunk / 16 + 1; i += kThreads) ((uint4*)stage)[i] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+    const long long c = uni(s_chunk);
+    if (c >= nchunks) break;
+    const long long base = c * kChunk;
+    const int osize = (int)min((long long)kChunk, a.insize - base);
+    const uint8_t* const src = a.in + base;
+
+    uint4 raw[kRounds];
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) raw[r] = load_slice(src, r * kThreads * 16 + t * 16, osize);
+
+    W code[kRounds][PS];
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) {
+      W v[PS];
+      slice_values(raw[r], v);
+      W orv = 0;
+#pragma unroll
+      for (int k = 0; k < PS; k++) {
+        code[r][k] = to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
+        orv |= code[r][k];
+      }
+      const int bl = half_wave_max(bit_length(orv));
+      if (lane32 == 0) ((uint8_t*)lgs)[r * 8 + half] = (uint8_t)bl;
+    }
+    __syncthreads();
+
+    uint32_t L[8];
+#pragma unroll
+    for (int i = 0; i < 8; i++) L[i] = uni(lgs[i]);
+    uint32_t nb = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) nb += byte_sum(L[i]);
+    const int total_bits = (int)nb * Word<W>::PerSub;
+    const int newsize = kSub + 2 + total_bits / 8;
+    const bool packed = newsize < osize;   // osize <= kChunk, so also newsize < kChunk
+    const int csize = packed ? newsize : osize;
+
+    if (t < 64) {
+      // wave 0: publish and look back for the chunk's stream offset
+      if (t == 0) {
+        flag_store(flags + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
+        sizes[c] = (uint16_t)csize;
+      }
+      const uint64_t excl = (c == 0) ? 0ull : look_back<false>(flags, c, nullptr);
+      if (t == 0) {
+        if (c != 0) flag_store(flags + c, kInc | (excl + (uint64_t)csize));
+        s_excl = excl;
+      }
+    }
+
+    if (packed) {
+      if (t < 8) stage[t] = L[t];                              // 32 bit-length bytes
+      if (t == 64) stage[8 + total_bits / 32] = (uint32_t)osize; // int16 trailer
+#pragma unroll
+      for (int r = 0; r < kRounds; r++) {
+        const int s = r * 8 + half;
+        const int ln = sub_length(L, s);
+        if (ln == 0) continue;
+        const int p0 = 8 * kSub + sub_bit_offset<W>(L, s) + lane32 * PS * ln;
+#pragma unroll
+        for (int k = 0; k < PS; k++) {
+          const W cv = code[r][k];
+          if (cv) or_code<W>(stage, p0 + k * ln, cv);
+        }
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < kRounds; r++) ((uint4*)stage)[r * kThreads + t] = raw[r];
+    }
+    __syncthreads();
+
+    const long long excl = uni((long long)s_excl);
+    store_unaligned(a.out + data_base + excl, stage, csize);
+    if (t == 0) {
+      if (c == 0) *(long long*)a.out = a.insize;
+      if (c == nchunks - 1) *a.outsize = data_base + excl + csize;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename W, bool LOSSY>
+__global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
+{
+  constexpr int PS = Word<W>::PerSlice;
+  constexpr int TB = Word<W>::TB;
+  __shared__ uint32_t stage[kStageBytes / 4];
+  __shared__ long long s_chunk;
+  __shared__ unsigned long long s_excl;
+
+  const int t = threadIdx.x;
+  const int lane32 = t & 31;
+  const int half = t >> 5;
+  const long long hdr = *(const long long*)a.in;
+  if (hdr < 0 || (a.cap >= 0 && hdr > a.cap)) {
+    if (blockIdx.x == 0 && t == 0) raise_status(a.ws, hdr < 0 ? kStCorrupt : kStCapacity);
+    return;
+  }
+  if (blockIdx.x == 0 && t == 0) *a.decsize = hdr;
+  const long long nchunks = (hdr + kChunk - 1) / kChunk;
+  const long long data_base = 8 + 2 * nchunks;
+  if (a.inlen >= 0 && data_base > a.inlen) {
+    if (blockIdx.x == 0 && t == 0) raise_status(a.ws, kStCorrupt);
+    return;
+  }
+  const uint16_t* const sizes = (const uint16_t*)(a.in + 8);
+  uint64_t* const flags = a.ws + kWsFlags;
+  const W offs = (W)a.offs;
+
+  for (;;) {
+    if (t == 0) s_chunk = (long long)take_ticket(a.ws);
+    __syncthreads();
+    const long long c = uni(s_chunk);
+    if (c >= nchunks) break;
+    const long long base = c * kChunk;
+    const int osize = (int)min((long long)kChunk, hdr - base);
+    const int csize = uni((int)sizes[c]);
+
+    if (t < 64) {
+      const uint64_t excl = (c == 0) ? 0ull : look_back<true>(flags, c, sizes);
+      if (t == 0) {
+        flag_store(flags + c, kInc | (excl + (uint64_t)csize));
+        s_excl = excl;
+      }
+    }
+    __syncthreads();
+    const long long excl = uni((long long)s_excl);
+    bool bad = csize > osize || (csize < osize && csize < kSub + 2);
+    if (a.inlen >= 0 && data_base + excl + csize > a.inlen) bad = true;
+
+    if (!bad) {
+      // aligned cover of the payload: payload byte i sits at stage byte mis + i
+      const uint8_t* const src = a.in + data_base + excl;
+      const int mis = (int)((uintptr_t)src & 15);
+      const uint4* const cover = (const uint4*)(src - mis);
+      const int nblk = (mis + csize + 15) >> 4;
+      for (int j = t; j < nblk; j += kThreads) ((uint4*)stage)[j] = cover[j];
+      __syncthreads();
+
+      uint8_t* const dst = a.out + base;
+      if (csize == osize) {
+#pragma unroll
+        for (int r = 0; r < kRounds; r++) {
+          const int off = r * kThreads * 16 + t * 16;
+          if (off < osize) store_slice(dst, off, osize, lds_read16(stage, mis + off));
+        }
+      } else {
+        // 32 bit-length bytes, realigned from the cover
+        uint32_t L[8];
+        const int q = mis >> 2;
+        const uint32_t sh = (uint32_t)(mis & 3) * 8u;
+        bool over = false;
+        uint32_t nb = 0;
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+          L[i] = uni(__builtin_amdgcn_alignbit(stage[q + i + 1], stage[q + i], sh));
+          over |= any_byte_above(L[i], (uint32_t)TB);
+          nb += byte_sum(L[i]);
+        }
+        const uint8_t* sb = (const uint8_t*)stage;
+        const int tr = uni((int)(int16_t)((uint16_t)sb[mis + csize - 2] | ((uint16_t)sb[mis + csize - 1] << 8)));
+        const bool corrupt = over || (long long)(kSub + 2) + (long long)nb * (Word<W>::PerSub / 8) != csize || tr != osize;
+        if (corrupt) {
+          if (t == 0) raise_status(a.ws, kStCorrupt);
+        } else {
+#pragma unroll
+          for (int r = 0; r < kRounds; r++) {
+            const int off = r * kThreads * 16 + t * 16;
+            const int s = r * 8 + half;
+            const int ln = sub_length(L, s);
+            const int p0 = 8 * (mis + kSub) + sub_bit_offset<W>(L, s) + lane32 * PS * ln;
+            W v[PS];
+#pragma unroll
+            for (int k = 0; k < PS; k++) v[k] = from_code<W, LOSSY>(extract_code(stage, p0 + k * ln, ln, (W*)nullptr), a.thr_e, offs);
+            if (off < osize) store_slice(dst, off, osize, values_slice(v));
+          }
+        }
+      }
+    } else if (t == 0) {
+      raise_status(a.ws, kStCorrupt);
+    }
+    __syncthreads();
+  }
+}
+
+// ---- launchers ----
+
+template <typename W, bool LOSSY>
+static hipError_t launch_encode(const EncodeArgs& a, int grid, hipStream_t stream)
+{
+  hipLaunchKernelGGL((encode_kernel<W, LOSSY>), dim3(grid), dim3(kThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+template <typename W, bool LOSSY>
+static hipError_t launch_decode(const DecodeArgs& a, int grid, hipStream_t stream)
+{
+  hipLaunchKernelGGL((decode_kernel<W, LOSSY>), dim3(grid), dim3(kThreads), 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, hipStream_t stream)
+{
+  if (elem_bytes == 4) return lossy ? launch_encode<uint32_t, true>(a, grid, stream) : launch_encode<uint32_t, false>(a, grid, stream);
+  return lossy ? launch_encode<uint64_t, true>(a, grid, stream) : launch_encode<uint64_t, false>(a, grid, stream);
+}
+
+hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream)
+{
+  if (elem_bytes == 4) return lossy ? launch_decode<uint32_t, true>(a, grid, stream) : launch_decode<uint32_t, false>(a, grid, stream);
+  return lossy ? launch_decode<uint64_t, true>(a, grid, stream) : launch_decode<uint64_t, false>(a, grid, stream);
+}
+
+int resident_grid(int elem_bytes, bool lossy, bool decode)
+{
+  int dev = 0, cus = 0, per = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+  const void* fn;
+  if (decode) {
+    fn = elem_bytes == 4 ? (lossy ? (const void*)decode_kernel<uint32_t, true> : (const void*)decode_kernel<uint32_t, false>)
+                         : (lossy ? (const void*)decode_kernel<uint64_t, true> : (const void*)decode_kernel<uint64_t, false>);
+  } else {
+    fn = elem_bytes == 4 ? (lossy ? (const void*)encode_kernel<uint32_t, true> : (const void*)encode_kernel<uint32_t, false>)
+                         : (lossy ? (const void*)encode_kernel<uint64_t, true> : (const void*)encode_kernel<uint64_t, false>);
+  }
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kThreads, 0) != hipSuccess) return 0;
+  return cus * (per > 0 ? per : 1);
+}
+
+}  // namespace sleek

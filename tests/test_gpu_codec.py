@@ -1,0 +1,172 @@
+"""Parity of the HIP codec with the oracle and the reference's golden streams.
+
+Every stream the GPU writes and every array it decodes must be byte-identical
+to the reference CPU codec's (golden fixtures) and to the oracle's on the same
+input; lossy output must also satisfy the reference's point-wise check
+(MAXABS_f32.h:40-95).  All GPU calls go through the C ABI of libsleek_amd.so.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from cases import build_input, eb_value
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "golden.json")))["cases"]
+
+
+def sha(b):
+    return hashlib.sha256(np.ascontiguousarray(b).view(np.uint8).tobytes()).hexdigest()
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def gpu_roundtrip(a, eb, blocks=0, sync=False):
+    """Compress + decompress a numpy array on the GPU; returns (stream, decoded) as numpy."""
+    import torch
+    from sleek_amd import codec
+
+    x = torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    if sync:
+        s = codec.compress_sync(x, eb, blocks=blocks)
+        torch.cuda.synchronize()
+        stream = s.cpu().numpy()
+        d = codec.decompress_sync(s, x.dtype, eb, blocks=blocks)
+    else:
+        out, n = codec.compress(x, eb, blocks=blocks)
+        ws = codec.Workspace(a.nbytes)
+        torch.cuda.synchronize()
+        stream = out[: int(n.item())].cpu().numpy()
+        d, dn = codec.decompress(out, x.dtype, eb, numel=a.size, blocks=blocks, workspace=ws)
+        assert ws.status() == 0
+        assert int(dn.item()) == a.nbytes
+    torch.cuda.synchronize()
+    return stream, d.cpu().numpy()
+
+
+def maxabs_ok(orig, recon, eb):
+    """The reference's point-wise check (MAXABS_f32.h:56-81), in the data's precision:
+    non-finite values must match (NaN matches NaN); finite ones need
+    |o - r| <= eb and o - eb <= r <= o + eb."""
+    o, r = orig, recon
+    fin = np.isfinite(o) & np.isfinite(r)
+    same = (o == r) | (np.isnan(o) & np.isnan(r))
+    if np.any(~fin & ~same):
+        return False
+    eb = o.dtype.type(eb)
+    of, rf = o[fin], r[fin]
+    with np.errstate(over="ignore", invalid="ignore"):
+        bad = (rf < of - eb) | (rf > of + eb) | (np.abs(of - rf) > eb)
+    return not np.any(bad)
+
+
+@pytest.mark.parametrize("case", GOLDEN, ids=[c["name"] for c in GOLDEN])
+def test_golden_streams(gpu, case):
+    say(f"golden {case['name']} n={case['n']}")
+    a = build_input(case)
+    assert sha(a) == case["input_sha256"], "input generator drifted"
+    eb = eb_value(case, a)
+    ebf = None if eb is None else float(eb)
+    stream, dec = gpu_roundtrip(a, ebf)
+    assert stream.size == case["stream_len"]
+    if case["stream_hex"] is not None:
+        assert stream.tobytes().hex() == case["stream_hex"]
+    assert sha(stream) == case["stream_sha256"]
+    assert sha(dec) == case["decoded_sha256"]
+    if eb is None:
+        assert dec.tobytes() == a.tobytes()
+    elif float(eb) > 0:
+        assert maxabs_ok(a, dec, eb)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("blocks", [1, 7, 0])
+def test_blocks_invariance(gpu, dt, blocks):
+    say(f"blocks={blocks} {np.dtype(dt).name}")
+    from sleek_amd.datagen import exact_field
+
+    a = exact_field(300_001 if dt == np.float32 else 150_001, 42, dt)
+    for eb in (None, 1e-3):
+        ref = po.compress(a, eb)
+        for sync in (False, True):
+            s, d = gpu_roundtrip(a, eb, blocks=blocks, sync=sync)
+            assert s.tobytes() == ref.tobytes()
+            od, rc = po.decompress(ref, dt, eb)
+            assert rc == 0 and d.tobytes() == od.tobytes()
+
+
+@pytest.mark.parametrize("config", [1, 2])
+def test_config_fields_scaled(gpu, config):
+    """The benchmark recipes (sinusoids, ramp, lognormal noise, specials) at 1/4 scale."""
+    say(f"config {config} at 1/4 scale")
+    from sleek_amd.datagen import config_field, finite_range
+
+    a = config_field(config, seed=3, scale=0.25).reshape(-1)
+    dt = a.dtype
+    rel = (1e-2, 1e-3, 1e-4) if dt == np.float32 else (1e-6, 1e-9)
+    r = finite_range(a)
+    for eb in (None,) + tuple(dt.type(r * dt.type(e)) for e in rel):
+        ebf = None if eb is None else float(eb)
+        s, d = gpu_roundtrip(a, ebf)
+        ref = po.compress(a, ebf)
+        assert s.tobytes() == ref.tobytes(), f"stream mismatch eb={ebf}"
+        od, rc = po.decompress(ref, dt, ebf)
+        assert rc == 0 and d.tobytes() == od.tobytes()
+        if ebf is None:
+            assert d.tobytes() == a.tobytes()
+        else:
+            assert maxabs_ok(a, d, eb)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_empty_and_tiny(gpu, dt):
+    say(f"empty/tiny {np.dtype(dt).name}")
+    for n in (0, 1, 2, 3, 5):
+        a = (np.arange(n) * 1.5 - 2).astype(dt)
+        for eb in (None, 1e-2):
+            s, d = gpu_roundtrip(a, eb)
+            assert s.tobytes() == po.compress(a, eb).tobytes()
+            if eb is None:
+                assert d.tobytes() == a.tobytes()
+
+
+def test_corrupt_stream_detected(gpu):
+    say("corrupt stream")
+    import torch
+    from sleek_amd import codec, _lib
+
+    a = np.linspace(0, 1, 4096 * 4, dtype=np.float32)
+    ref = po.compress(a)
+    # damage the trailer of chunk 0 (compressed: size < 16384)
+    sizes = ref[8:8 + 2 * 4].view(np.uint16)
+    assert sizes[0] < 16384
+    bad = ref.copy()
+    base = 8 + 2 * 4
+    bad[base + int(sizes[0]) - 2] ^= 0x5A
+    s = torch.from_numpy(bad).cuda()
+    ws = codec.Workspace(a.nbytes)
+    codec.decompress(s, torch.float32, numel=a.size, workspace=ws)
+    assert ws.status() == _lib.ECORRUPT
+    # a size-table entry larger than the chunk
+    bad = ref.copy()
+    bad[8:10] = np.frombuffer(np.uint16(20000).tobytes(), np.uint8)
+    s = torch.from_numpy(bad).cuda()
+    codec.decompress(s, torch.float32, numel=a.size, workspace=ws, enclen=bad.size)
+    assert ws.status() == _lib.ECORRUPT
+    # decoded length above the capacity
+    s = torch.from_numpy(ref).cuda()
+    codec.decompress(s, torch.float32, numel=a.size - 1, workspace=ws)
+    assert ws.status() == _lib.ECAPACITY
+    # a clean decode after the failures
+    d, _ = codec.decompress(s, torch.float32, numel=a.size, workspace=ws)
+    assert ws.status() == 0
+    assert d.cpu().numpy().tobytes() == a.tobytes()

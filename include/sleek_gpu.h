@@ -101,6 +101,13 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
                                const unsigned char* d_encoded, long long enclen, void* d_decoded,
                                long long capacity, long long* d_decsize, void* d_workspace,
                                long long workspace_bytes, int blocks, void* stream);
+/* Launch-shape knobs, found by the autotuner (sleek_amd/autotune.py); no
+ * reference counterpart.  encode_waves: 8 = register budget for 8 waves per
+ * SIMD, any other positive value = compiler's choice; encode_grid /
+ * decode_grid: persistent grid sizes (0 = automatic).  They change speed
+ * only, never the bytes.  Applies to calls made after it returns. */
+int sleek_gpu_set_tuning(int encode_waves, int encode_grid, int decode_grid);
+
 /* Synchronises `stream`, then returns SLEEK_OK, SLEEK_ECORRUPT or
  * SLEEK_ECAPACITY from the workspace's status word. */
 int sleek_gpu_workspace_status(const void* d_workspace, void* stream);

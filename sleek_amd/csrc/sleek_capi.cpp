@@ -27,7 +27,14 @@ int hip_fail(hipError_t e, const char* what)
 
 long long chunks_of(long long bytes) { return (bytes + SLEEK_CHUNK_BYTES - 1) / SLEEK_CHUNK_BYTES; }
 
-long long ws_bytes_for(long long insize) { return (8 + chunks_of(insize)) * 8; }
+// workspace: 8 header words (ticket, status), then one word per chunk and
+// a total (look-back words when encoding, offsets when decoding)
+long long ws_bytes_for(long long insize) { return (16 + chunks_of(insize)) * 8; }
+
+// tuning knobs (sleek_gpu_set_tuning); none changes the bytes produced
+int g_encode_waves = 8;
+int g_encode_grid = 0;
+int g_decode_grid = 0;
 
 // Per-device scratch of the synchronous entry points (grow-only).
 struct DeviceScratch {
@@ -62,7 +69,9 @@ int scratch_for(long long need, void** out)
 
 int grid_for(int blocks, int elem_bytes, bool lossy, bool decode, long long nchunks)
 {
-  long long g = blocks > 0 ? blocks : sleek::resident_grid(elem_bytes, lossy, decode);
+  long long g = blocks;
+  if (g <= 0) g = decode ? g_decode_grid : g_encode_grid;
+  if (g <= 0) g = decode ? nchunks : sleek::resident_grid(elem_bytes, lossy, false, g_encode_waves);
   if (g <= 0) g = 1024;
   if (nchunks >= 0 && g > nchunks) g = nchunks > 0 ? nchunks : 1;
   if (g > (1 << 20)) g = 1 << 20;
@@ -92,7 +101,7 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   a.thr_e = thr_e;
   a.offs = offs;
   const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
-  e = sleek::encode_launch(elem_bytes, lossy, a, grid, stream);
+  e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_waves, stream);
   if (e != hipSuccess) return hip_fail(e, "encode launch");
   return SLEEK_OK;
 }
@@ -109,9 +118,10 @@ int decompress_impl(int elem_bytes, bool lossy, int thr_e, unsigned long long of
   }
   // without a capacity the workspace bounds the chunk count
   const long long ws_chunks = ws_bytes / 8 - 8;
-  if (ws_chunks < 0) return SLEEK_EWORKSPACE;
-  const long long cap = capacity >= 0 ? capacity : ws_chunks * SLEEK_CHUNK_BYTES;
-  hipError_t e = hipMemsetAsync(ws, 0, (size_t)ws_bytes_for(cap), stream);
+  const long long cap = capacity >= 0 ? capacity : (ws_chunks - 8) * SLEEK_CHUNK_BYTES;
+  if (cap < 0) return SLEEK_EWORKSPACE;
+  // only the ticket and status words need clearing: the offsets are written
+  hipError_t e = hipMemsetAsync(ws, 0, 64, stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
   sleek::DecodeArgs a;
   a.in = d_encoded;
@@ -195,7 +205,7 @@ int sleek_gpu_system_checks(void)
 
 int sleek_gpu_thread_blocks(void)
 {
-  const int g = sleek::resident_grid(4, false, false);
+  const int g = sleek::resident_grid(4, false, false, g_encode_waves);
   return g > 0 ? g : SLEEK_EHIP;
 }
 
@@ -288,6 +298,16 @@ int sleek_gpu_decompress_lossy_f64(int blocks, int eb_e, int thr_e, unsigned lon
 {
   (void)eb_e;
   return decompress_sync(8, true, thr_e, offs_e, d_encoded, d_decoded, d_decsize, blocks);
+}
+
+int sleek_gpu_set_tuning(int encode_waves, int encode_grid, int decode_grid)
+{
+  if (encode_waves < 0 || encode_grid < 0 || decode_grid < 0) return SLEEK_EINVAL;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_encode_waves = encode_waves > 0 ? encode_waves : 8;
+  g_encode_grid = encode_grid;
+  g_decode_grid = decode_grid;
+  return SLEEK_OK;
 }
 
 long long sleek_gpu_workspace_size(long long max_insize)

@@ -191,6 +191,14 @@ __device__ __forceinline__ long long uni(long long v)
   return (long long)(((unsigned long long)hi << 32) | lo);
 }
 
+// Make the compiler treat q as redefined here (an empty asm with register
+// operands), so values derived from q before this point are recomputed
+// rather than kept live in registers.
+__device__ __forceinline__ void reissue(uint4& q)
+{
+  asm volatile("" : "+v"(q.x), "+v"(q.y), "+v"(q.z), "+v"(q.w));
+}
+
 // Any byte of x above n (n <= 127), without carries between bytes.
 __device__ __forceinline__ bool any_byte_above(uint32_t x, uint32_t n)
 {
@@ -243,11 +251,8 @@ __device__ __forceinline__ void raise_status(uint64_t* ws, uint64_t bits)
 // Called by one whole wave; returns the exclusive prefix of chunk c (the sum
 // of the sizes of chunks 0..c-1) in every lane.  Each lane watches one
 // predecessor of a 64-wide window that slides back until it meets an
-// inclusive word.  With DECODE the sizes are already known from the size
-// table, so an unpublished predecessor contributes its table entry and the
-// wave never waits.
-template <bool DECODE>
-__device__ __forceinline__ uint64_t look_back(const uint64_t* flags, long long c, const uint16_t* sizes)
+// inclusive word; the wave waits while a needed predecessor is unpublished.
+__device__ __forceinline__ uint64_t look_back(const uint64_t* flags, long long c)
 {
   const int lane = threadIdx.x & 63;
   uint64_t excl = 0;
@@ -255,11 +260,7 @@ __device__ __forceinline__ uint64_t look_back(const uint64_t* flags, long long c
   while (end > 0) {
     const long long j = end - 1 - lane;
     while (true) {
-      uint64_t f = kInc;   // before chunk 0: inclusive 0
-      if (j >= 0) {
-        f = flag_load(flags + j);
-        if (DECODE && (f >> 62) == 0) f = kAgg | (uint64_t)sizes[j];
-      }
+      const uint64_t f = (j >= 0) ? flag_load(flags + j) : kInc;   // before chunk 0: inclusive 0
       const uint64_t inc = __ballot((f >> 62) == 2);
       const uint64_t rdy = __ballot((f >> 62) != 0);
       if (inc) {

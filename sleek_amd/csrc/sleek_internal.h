@@ -27,8 +27,11 @@ struct DecodeArgs {
   unsigned long long offs;
 };
 
-hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, hipStream_t stream);
+// waves: occupancy target of the encoder (8 = register budget of 8 waves per
+// SIMD; anything lower = the compiler's choice).  Output bytes never depend
+// on it or on the grid size.
+hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int waves, hipStream_t stream);
 hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream);
-int resident_grid(int elem_bytes, bool lossy, bool decode);
+int resident_grid(int elem_bytes, bool lossy, bool decode, int waves);
 
 }  // namespace sleek

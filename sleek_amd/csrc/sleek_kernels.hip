@@ -62,15 +62,20 @@ __device__ __forceinline__ uint64_t extract_code(const uint32_t* st, int b, int 
   return ln >= 64 ? v : (v & ((1ull << ln) - 1ull));
 }
 
-// Load slice r of a chunk of osize bytes (zero beyond osize).
+// Load the 16-byte slice at byte off of a chunk of osize bytes, zero beyond
+// osize.  The slice is 16-byte aligned, so when any of its bytes is valid the
+// whole slice lies in a mapped page and can be loaded at once.
 __device__ __forceinline__ uint4 load_slice(const uint8_t* src, int off, int osize)
 {
-  if (off + 16 <= osize) return *(const uint4*)(src + off);
-  uint32_t d[4] = {0u, 0u, 0u, 0u};
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-    if (off + 4 * i + 4 <= osize) d[i] = ((const uint32_t*)(src + off))[i];
-  return make_uint4(d[0], d[1], d[2], d[3]);
+  if (off >= osize) return make_uint4(0u, 0u, 0u, 0u);
+  uint4 q = *(const uint4*)(src + off);
+  const int valid = osize - off;   // bytes; a multiple of 4
+  if (valid < 16) {
+    q.w = 0u;
+    if (valid < 12) q.z = 0u;
+    if (valid < 8) q.y = 0u;
+  }
+  return q;
 }
 
 __device__ __forceinline__ void store_slice(uint8_t* dst, int off, int osize, uint4 q)
@@ -85,13 +90,13 @@ __device__ __forceinline__ void store_slice(uint8_t* dst, int off, int osize, ui
     if (off + 4 * i + 4 <= osize) ((uint32_t*)(dst + off))[i] = d[i];
 }
 
-template <typename W, bool LOSSY>
-__global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a)
+template <typename W, bool LOSSY, int WAVES>
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void encode_kernel(EncodeArgs a)
 {
   constexpr int PS = Word<W>::PerSlice;
   __shared__ uint32_t stage[kStageBytes / 4];
   __shared__ uint32_t lgs[8];
-  __shared__ long long s_chunk;
+  __shared__ long long s_ticket[2];
   __shared__ unsigned long long s_excl;
 
   const int t = threadIdx.x;
@@ -111,36 +116,42 @@ __global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a)
     return;
   }
 
-  for (;;) {
-    if (t == 0) s_chunk = (long long)take_ticket(a.ws);
+  // Tickets run one chunk ahead: the next one is taken while the current
+  // chunk is coded, so the atomic's latency is off the critical path.  A
+  // block only ever holds tickets above the one it is coding, so the
+  // smallest unfinished chunk always belongs to a block that is coding it.
+  if (t == 0) s_ticket[0] = (long long)take_ticket(a.ws);
+  for (int it = 0;; it++) {
     for (int i = t; i < kChunk / 16 + 1; i += kThreads) ((uint4*)stage)[i] = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
-    const long long c = uni(s_chunk);
+    const long long c = uni(s_ticket[it & 1]);
     if (c >= nchunks) break;
     const long long base = c * kChunk;
     const int osize = (int)min((long long)kChunk, a.insize - base);
     const uint8_t* const src = a.in + base;
 
+    // Only the raw slices stay live across the barriers (16 VGPRs); the
+    // codes are cheap and are computed again in the packing pass.
     uint4 raw[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; r++) raw[r] = load_slice(src, r * kThreads * 16 + t * 16, osize);
+    long long next = 0;
+    if (t == 0) next = (long long)take_ticket(a.ws);
 
-    W code[kRounds][PS];
 #pragma unroll
     for (int r = 0; r < kRounds; r++) {
       W v[PS];
       slice_values(raw[r], v);
       W orv = 0;
 #pragma unroll
-      for (int k = 0; k < PS; k++) {
-        code[r][k] = to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
-        orv |= code[r][k];
-      }
+      for (int k = 0; k < PS; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
       const int bl = half_wave_max(bit_length(orv));
       if (lane32 == 0) ((uint8_t*)lgs)[r * 8 + half] = (uint8_t)bl;
     }
     __syncthreads();
 
+#pragma unroll
+    for (int r = 0; r < kRounds; r++) reissue(raw[r]);
     uint32_t L[8];
 #pragma unroll
     for (int i = 0; i < 8; i++) L[i] = uni(lgs[i]);
@@ -152,37 +163,44 @@ __global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a)
     const bool packed = newsize < osize;   // osize <= kChunk, so also newsize < kChunk
     const int csize = packed ? newsize : osize;
 
-    if (t < 64) {
-      // wave 0: publish and look back for the chunk's stream offset
-      if (t == 0) {
-        flag_store(flags + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
-        sizes[c] = (uint16_t)csize;
-      }
-      const uint64_t excl = (c == 0) ? 0ull : look_back<false>(flags, c, nullptr);
-      if (t == 0) {
-        if (c != 0) flag_store(flags + c, kInc | (excl + (uint64_t)csize));
-        s_excl = excl;
-      }
+    if (t == 0) {
+      // publish the chunk's size at once: successors can sum it while this
+      // block packs and looks back
+      flag_store(flags + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
+      sizes[c] = (uint16_t)csize;
     }
 
     if (packed) {
-      if (t < 8) stage[t] = L[t];                              // 32 bit-length bytes
-      if (t == 64) stage[8 + total_bits / 32] = (uint32_t)osize; // int16 trailer
+      if (t < 8) stage[t] = L[t];                                // 32 bit-length bytes
+      if (t == 64) stage[8 + total_bits / 32] = (uint32_t)osize;  // int16 trailer
 #pragma unroll
       for (int r = 0; r < kRounds; r++) {
         const int s = r * 8 + half;
         const int ln = sub_length(L, s);
         if (ln == 0) continue;
         const int p0 = 8 * kSub + sub_bit_offset<W>(L, s) + lane32 * PS * ln;
+        W v[PS];
+        slice_values(raw[r], v);
 #pragma unroll
         for (int k = 0; k < PS; k++) {
-          const W cv = code[r][k];
+          const W cv = to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
           if (cv) or_code<W>(stage, p0 + k * ln, cv);
         }
       }
     } else {
+      // raw chunk: the original bytes
 #pragma unroll
       for (int r = 0; r < kRounds; r++) ((uint4*)stage)[r * kThreads + t] = raw[r];
+    }
+
+    if (t < 64) {
+      // wave 0, after its share of the packing: the chunk's stream offset
+      const uint64_t excl = (c == 0) ? 0ull : look_back(flags, c);
+      if (t == 0) {
+        if (c != 0) flag_store(flags + c, kInc | (excl + (uint64_t)csize));
+        s_excl = excl;
+        s_ticket[(it + 1) & 1] = next;
+      }
     }
     __syncthreads();
 
@@ -192,58 +210,106 @@ __global__ __launch_bounds__(kThreads) void encode_kernel(EncodeArgs a)
       if (c == 0) *(long long*)a.out = a.insize;
       if (c == nchunks - 1) *a.outsize = data_base + excl + csize;
     }
-    __syncthreads();
+    __syncthreads();   // stage is cleared for the next chunk only after every store read it
   }
 }
 
+// Validity of a stream header against the caller's limits (uniform).
+__device__ __forceinline__ bool header_ok(const DecodeArgs& a, long long hdr, long long* nchunks, uint64_t* why)
+{
+  if (hdr < 0) { *why = kStCorrupt; return false; }
+  if (a.cap >= 0 && hdr > a.cap) { *why = kStCapacity; return false; }
+  *nchunks = (hdr + kChunk - 1) / kChunk;
+  if (a.inlen >= 0 && 8 + 2 * *nchunks > a.inlen) { *why = kStCorrupt; return false; }
+  return true;
+}
+
+// Exclusive prefix sums of the uint16 size table into the workspace (one
+// int64 per chunk plus the total), by one workgroup of 1024 threads sweeping
+// the table in tiles of 8192 entries.  Also writes *decsize.
+constexpr int kScanThreads = 1024;
+__global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
+{
+  __shared__ long long wsum[kScanThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const long long hdr = uni(*(const long long*)a.in);
+  long long nchunks = 0;
+  uint64_t why = 0;
+  if (!header_ok(a, hdr, &nchunks, &why)) {
+    if (t == 0) raise_status(a.ws, why);
+    return;
+  }
+  if (t == 0) *a.decsize = hdr;
+  const uint8_t* const tab = a.in + 8;
+  long long* const offs = (long long*)(a.ws + kWsFlags);
+  long long carry = 0;
+  for (long long base = 0; base < nchunks; base += (long long)kScanThreads * 8) {
+    const long long i0 = base + (long long)t * 8;
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = (i0 + k < nchunks) ? (uint32_t)*(const uint16_t*)(tab + 2 * (i0 + k)) : 0u;
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) local += v[k];
+    // inclusive wave scan
+    uint32_t inc = local;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t y = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += y;
+    }
+    if (lane == 63) wsum[wave] = inc;
+    __syncthreads();
+    long long before = 0, tile = 0;
+#pragma unroll
+    for (int w = 0; w < kScanThreads / 64; w++) {
+      const long long x = wsum[w];
+      if (w < wave) before += x;
+      tile += x;
+    }
+    long long run = carry + before + (long long)(inc - local);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (i0 + k < nchunks) offs[i0 + k] = run;
+      run += v[k];
+    }
+    carry += tile;
+    __syncthreads();
+  }
+  if (t == 0) offs[nchunks] = carry;
+}
+
+// Chunk decode: no cross-workgroup dependency once the offsets are known,
+// so workgroups stride over chunks in any order.
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 {
   constexpr int PS = Word<W>::PerSlice;
   constexpr int TB = Word<W>::TB;
   __shared__ uint32_t stage[kStageBytes / 4];
-  __shared__ long long s_chunk;
-  __shared__ unsigned long long s_excl;
 
   const int t = threadIdx.x;
   const int lane32 = t & 31;
   const int half = t >> 5;
-  const long long hdr = *(const long long*)a.in;
-  if (hdr < 0 || (a.cap >= 0 && hdr > a.cap)) {
-    if (blockIdx.x == 0 && t == 0) raise_status(a.ws, hdr < 0 ? kStCorrupt : kStCapacity);
-    return;
-  }
-  if (blockIdx.x == 0 && t == 0) *a.decsize = hdr;
-  const long long nchunks = (hdr + kChunk - 1) / kChunk;
+  const long long hdr = uni(*(const long long*)a.in);
+  long long nchunks = 0;
+  uint64_t why = 0;
+  if (!header_ok(a, hdr, &nchunks, &why)) return;   // reported by offsets_kernel
   const long long data_base = 8 + 2 * nchunks;
-  if (a.inlen >= 0 && data_base > a.inlen) {
+  const uint16_t* const sizes = (const uint16_t*)(a.in + 8);
+  const long long* const offs_tab = (const long long*)(a.ws + kWsFlags);
+  const W offs = (W)a.offs;
+  if (a.inlen >= 0 && data_base + uni(offs_tab[nchunks]) > a.inlen) {
     if (blockIdx.x == 0 && t == 0) raise_status(a.ws, kStCorrupt);
     return;
   }
-  const uint16_t* const sizes = (const uint16_t*)(a.in + 8);
-  uint64_t* const flags = a.ws + kWsFlags;
-  const W offs = (W)a.offs;
 
-  for (;;) {
-    if (t == 0) s_chunk = (long long)take_ticket(a.ws);
-    __syncthreads();
-    const long long c = uni(s_chunk);
-    if (c >= nchunks) break;
+  for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) {
     const long long base = c * kChunk;
     const int osize = (int)min((long long)kChunk, hdr - base);
     const int csize = uni((int)sizes[c]);
-
-    if (t < 64) {
-      const uint64_t excl = (c == 0) ? 0ull : look_back<true>(flags, c, sizes);
-      if (t == 0) {
-        flag_store(flags + c, kInc | (excl + (uint64_t)csize));
-        s_excl = excl;
-      }
-    }
-    __syncthreads();
-    const long long excl = uni((long long)s_excl);
-    bool bad = csize > osize || (csize < osize && csize < kSub + 2);
-    if (a.inlen >= 0 && data_base + excl + csize > a.inlen) bad = true;
+    const long long excl = uni(offs_tab[c]);
+    const bool bad = csize > osize || (csize < osize && csize < kSub + 2);
 
     if (!bad) {
       // aligned cover of the payload: payload byte i sits at stage byte mis + i
@@ -303,23 +369,25 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 // ---- launchers ----
 
 template <typename W, bool LOSSY>
-static hipError_t launch_encode(const EncodeArgs& a, int grid, hipStream_t stream)
+static hipError_t launch_encode(const EncodeArgs& a, int grid, int waves, hipStream_t stream)
 {
-  hipLaunchKernelGGL((encode_kernel<W, LOSSY>), dim3(grid), dim3(kThreads), 0, stream, a);
+  if (waves >= 8) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 8>), dim3(grid), dim3(kThreads), 0, stream, a);
+  else hipLaunchKernelGGL((encode_kernel<W, LOSSY, 1>), dim3(grid), dim3(kThreads), 0, stream, a);
   return hipGetLastError();
 }
 
 template <typename W, bool LOSSY>
 static hipError_t launch_decode(const DecodeArgs& a, int grid, hipStream_t stream)
 {
+  hipLaunchKernelGGL(offsets_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
   hipLaunchKernelGGL((decode_kernel<W, LOSSY>), dim3(grid), dim3(kThreads), 0, stream, a);
   return hipGetLastError();
 }
 
-hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, hipStream_t stream)
+hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int waves, hipStream_t stream)
 {
-  if (elem_bytes == 4) return lossy ? launch_encode<uint32_t, true>(a, grid, stream) : launch_encode<uint32_t, false>(a, grid, stream);
-  return lossy ? launch_encode<uint64_t, true>(a, grid, stream) : launch_encode<uint64_t, false>(a, grid, stream);
+  if (elem_bytes == 4) return lossy ? launch_encode<uint32_t, true>(a, grid, waves, stream) : launch_encode<uint32_t, false>(a, grid, waves, stream);
+  return lossy ? launch_encode<uint64_t, true>(a, grid, waves, stream) : launch_encode<uint64_t, false>(a, grid, waves, stream);
 }
 
 hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream)
@@ -328,7 +396,13 @@ hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int gr
   return lossy ? launch_decode<uint64_t, true>(a, grid, stream) : launch_decode<uint64_t, false>(a, grid, stream);
 }
 
-int resident_grid(int elem_bytes, bool lossy, bool decode)
+template <typename W, bool LOSSY>
+static const void* encode_fn(int waves)
+{
+  return waves >= 8 ? (const void*)encode_kernel<W, LOSSY, 8> : (const void*)encode_kernel<W, LOSSY, 1>;
+}
+
+int resident_grid(int elem_bytes, bool lossy, bool decode, int waves)
 {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -338,8 +412,8 @@ int resident_grid(int elem_bytes, bool lossy, bool decode)
     fn = elem_bytes == 4 ? (lossy ? (const void*)decode_kernel<uint32_t, true> : (const void*)decode_kernel<uint32_t, false>)
                          : (lossy ? (const void*)decode_kernel<uint64_t, true> : (const void*)decode_kernel<uint64_t, false>);
   } else {
-    fn = elem_bytes == 4 ? (lossy ? (const void*)encode_kernel<uint32_t, true> : (const void*)encode_kernel<uint32_t, false>)
-                         : (lossy ? (const void*)encode_kernel<uint64_t, true> : (const void*)encode_kernel<uint64_t, false>);
+    fn = elem_bytes == 4 ? (lossy ? encode_fn<uint32_t, true>(waves) : encode_fn<uint32_t, false>(waves))
+                         : (lossy ? encode_fn<uint64_t, true>(waves) : encode_fn<uint64_t, false>(waves));
   }
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kThreads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);

@@ -1,0 +1,63 @@
+"""Structure of the compiled kernels.
+
+Every workgroup barrier of every kernel must sit in the kernel's single
+outer loop (never inside a nested, lane-masked loop), and each kernel has the
+barrier count its design needs.  A barrier inside a divergent inner loop can
+leave lanes spinning on stale shared state; this check reads the gfx950
+assembly hipcc emits for the codec."""
+import os
+import re
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "sleek_amd", "csrc", "sleek_kernels.hip")
+
+
+def kernel_barriers(asm):
+    fn, depth, out = None, 0, {}
+    lines = asm.split("\n")
+    for i, l in enumerate(lines):
+        m = re.match(r"^(_Z\w+):", l)
+        if m:
+            fn, depth = m.group(1), 0
+            continue
+        if re.match(r"^(\.LBB\d+_\d+|; %bb\.\d+):", l):
+            depth = 0
+            j = i
+            while j < len(lines):
+                mm = re.search(r"Depth=(\d+)", lines[j])
+                if mm:
+                    depth = max(depth, int(mm.group(1)))
+                j += 1
+                if j >= len(lines) or not lines[j].strip().startswith(";"):
+                    break
+        if "s_barrier" in l and fn:
+            out.setdefault(fn, []).append(depth)
+    return out
+
+
+@pytest.fixture(scope="module")
+def asm(tmp_path_factory):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    d = tmp_path_factory.mktemp("asm")
+    out = d / "k.s"
+    subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-S",
+                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", SRC, "-o", str(out)],
+                   check=True, capture_output=True)
+    return out.read_text()
+
+
+def test_barriers_in_single_outer_loop(asm):
+    b = kernel_barriers(asm)
+    enc = [k for k in b if "encode_kernel" in k]
+    dec = [k for k in b if "decode_kernel" in k]
+    assert len(enc) == 8 and len(dec) == 4
+    for k in enc:
+        assert b[k] == [1, 1, 1, 1], (k, b[k])   # tickets, bit lengths, payload ready, stage reuse
+    for k in dec:
+        assert b[k] == [1, 1], (k, b[k])          # payload cover loaded, stage reuse

@@ -104,8 +104,8 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
 /* Launch-shape knobs, found by the autotuner (sleek_amd/autotune.py); no
  * reference counterpart.  encode_waves: 8 = register budget for 8 waves per
  * SIMD, any other positive value = compiler's choice; encode_grid /
- * decode_grid: persistent grid sizes (0 = automatic).  They change speed
- * only, never the bytes.  Applies to calls made after it returns. */
+ * decode_grid: grid sizes (0 = automatic).  They change speed only, never
+ * the bytes.  Applies to calls made after it returns. */
 int sleek_gpu_set_tuning(int encode_waves, int encode_grid, int decode_grid);
 
 /* Synchronises `stream`, then returns SLEEK_OK, SLEEK_ECORRUPT or

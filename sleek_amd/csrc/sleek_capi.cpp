@@ -27,9 +27,14 @@ int hip_fail(hipError_t e, const char* what)
 
 long long chunks_of(long long bytes) { return (bytes + SLEEK_CHUNK_BYTES - 1) / SLEEK_CHUNK_BYTES; }
 
-// workspace: 8 header words (ticket, status), then one word per chunk and
-// a total (look-back words when encoding, offsets when decoding)
-long long ws_bytes_for(long long insize) { return (16 + chunks_of(insize)) * 8; }
+// workspace: 8 header words (word 1: status), then per chunk one word, and
+// per group of 64 chunks two words (look-back state when encoding; chunk
+// offsets and their total when decoding)
+long long ws_bytes_for(long long insize)
+{
+  const long long ch = chunks_of(insize);
+  return (16 + ch + 2 * ((ch + 63) / 64)) * 8;
+}
 
 // tuning knobs (sleek_gpu_set_tuning); none changes the bytes produced
 int g_encode_waves = 8;
@@ -120,7 +125,7 @@ int decompress_impl(int elem_bytes, bool lossy, int thr_e, unsigned long long of
   const long long ws_chunks = ws_bytes / 8 - 8;
   const long long cap = capacity >= 0 ? capacity : (ws_chunks - 8) * SLEEK_CHUNK_BYTES;
   if (cap < 0) return SLEEK_EWORKSPACE;
-  // only the ticket and status words need clearing: the offsets are written
+  // only the status word needs clearing: the offsets are written
   hipError_t e = hipMemsetAsync(ws, 0, 64, stream);
   if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
   sleek::DecodeArgs a;

@@ -20,13 +20,16 @@ constexpr int kRounds = kChunk / (kThreads * 16);   // 16-byte slices per thread
 constexpr int kStageBytes = kChunk + 64;            // LDS staging buffer (payload + realignment slack)
 
 // ---- scan state words ----
-// Workspace layout (uint64 words): [0] ticket, [1] status bits, [8..) one
-// look-back word per chunk.  A look-back word holds the value in bits 61..0
-// and its kind in bits 63..62: 0 = not published, 1 = aggregate (the
-// chunk's own compressed size), 2 = inclusive (offset just past the chunk).
-constexpr int kWsTicket = 0;
+// Workspace layout (uint64 words): [1] status bits, then (from
+// word 8) one look-back word per chunk, one per group of kGroup chunks, and
+// one completion counter per group.  A look-back word holds a byte count in
+// bits 61..0 and its kind in bits 63..62: 0 = not published, 1 = aggregate
+// (the chunk's or group's own compressed size), 2 = inclusive (stream offset
+// just past it).  Kinds only ever increase, so inclusive words are written
+// with an atomic max and are never downgraded.
 constexpr int kWsStatus = 1;
 constexpr int kWsFlags = 8;
+constexpr int kGroup = 64;
 constexpr uint64_t kAgg = 1ull << 62;
 constexpr uint64_t kInc = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 62) - 1;
@@ -238,47 +241,53 @@ __device__ __forceinline__ void flag_store(uint64_t* p, uint64_t v)
 {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ unsigned long long take_ticket(uint64_t* ws)
-{
-  return __hip_atomic_fetch_add((unsigned long long*)&ws[kWsTicket], 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
 __device__ __forceinline__ void raise_status(uint64_t* ws, uint64_t bits)
 {
   __hip_atomic_fetch_or((unsigned long long*)&ws[kWsStatus], (unsigned long long)bits, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Decoupled look-back over the chunk-order prefix of compressed sizes.
-// Called by one whole wave; returns the exclusive prefix of chunk c (the sum
-// of the sizes of chunks 0..c-1) in every lane.  Each lane watches one
-// predecessor of a 64-wide window that slides back until it meets an
-// inclusive word; the wave waits while a needed predecessor is unpublished.
-__device__ __forceinline__ uint64_t look_back(const uint64_t* flags, long long c)
+__device__ __forceinline__ void flag_max(uint64_t* p, uint64_t v)
 {
-  const int lane = threadIdx.x & 63;
-  uint64_t excl = 0;
-  long long end = c;
-  while (end > 0) {
-    const long long j = end - 1 - lane;
-    while (true) {
-      const uint64_t f = (j >= 0) ? flag_load(flags + j) : kInc;   // before chunk 0: inclusive 0
-      const uint64_t inc = __ballot((f >> 62) == 2);
-      const uint64_t rdy = __ballot((f >> 62) != 0);
-      if (inc) {
-        const int p = __ffsll((unsigned long long)inc) - 1;
-        const uint64_t need = (p == 63) ? ~0ull : ((2ull << p) - 1);
-        if ((rdy & need) == need) {
-          excl += wave_sum(lane <= p ? (f & kValMask) : 0);
-          return excl;
-        }
-      } else if (rdy == ~0ull) {
-        excl += wave_sum(f & kValMask);
-        end -= 64;
-        break;
-      }
-      __builtin_amdgcn_s_sleep(1);
-    }
-  }
-  return excl;
+  __hip_atomic_fetch_max((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Scan state of one encode call inside its workspace.
+struct ScanState {
+  uint64_t* chunk;               // one word per chunk
+  uint64_t* group;               // one word per group of kGroup chunks
+  unsigned long long* count;     // per group: (chunks reported << 32) | bytes
+};
+
+__device__ __forceinline__ ScanState scan_state(uint64_t* ws, long long nchunks)
+{
+  const long long ngroups = (nchunks + kGroup - 1) / kGroup;
+  ScanState s;
+  s.chunk = ws + kWsFlags;
+  s.group = s.chunk + nchunks;
+  s.count = (unsigned long long*)(s.group + ngroups);
+  return s;
+}
+
+// Publish chunk c's compressed size: its own aggregate word, and the group's
+// aggregate when c is the last of its group to report.  Only the chunk's own
+// workgroup calls this (each chunk counts once in its group's counter).
+__device__ __forceinline__ void publish_size(const ScanState& s, long long c, long long nchunks, uint32_t csize)
+{
+  flag_max(s.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
+  const long long g = c / kGroup;
+  const unsigned long long old = __hip_atomic_fetch_add(s.count + g, (1ull << 32) | csize, __ATOMIC_RELAXED,
+                                                        __HIP_MEMORY_SCOPE_AGENT);
+  const long long members = min((long long)kGroup, nchunks - g * kGroup);
+  if ((long long)(old >> 32) + 1 == members)
+    flag_max(s.group + g, (g == 0 ? kInc : kAgg) | ((old & 0xffffffffull) + csize));
+}
+
+// Publish chunk c's inclusive offset (and its group's, for the group's last chunk).
+__device__ __forceinline__ void publish_inclusive(const ScanState& s, long long c, long long nchunks, uint64_t incl)
+{
+  flag_max(s.chunk + c, kInc | incl);
+  const long long g = c / kGroup;
+  if ((c % kGroup) == kGroup - 1 || c == nchunks - 1) flag_max(s.group + g, kInc | incl);
 }
 
 // 16 bytes starting at an arbitrary byte offset of a dword-aligned LDS buffer

@@ -11,7 +11,7 @@ struct EncodeArgs {
   long long insize;       // bytes
   uint8_t* out;           // stream (8-byte aligned, >= upper-bound size)
   long long* outsize;     // device: stream length in bytes
-  uint64_t* ws;           // zeroed workspace: ticket, status, look-back words
+  uint64_t* ws;           // zeroed workspace: status and look-back words
   int eb_e, thr_e;        // lossy parameters (ignored for lossless)
   unsigned long long offs;
 };

@@ -1,16 +1,17 @@
 // SLEEK encode / decode kernels for gfx950.
 //
-// One workgroup (4 wave64s) codes one 16 KiB chunk at a time and loops over
-// chunk tickets until the array is done (persistent grid).  Per chunk:
-//   encode: 16-byte coalesced loads (4 slices per thread) -> per-value code in
-//           registers -> per-subchunk bit length by a half-wave max ->
-//           compressed/raw decision -> publish the size and run a decoupled
-//           look-back (wave 0) while the other waves OR-pack the codes into
-//           an LDS staging buffer -> realigned 16-byte stores to the stream.
-//   decode: look-back over the size table (never waits: unpublished
-//           predecessors contribute their table entry) -> aligned 16-byte
-//           loads of the payload's cover into LDS -> per-lane bit extraction
-//           -> inverse transform -> 16-byte coalesced stores.
+// encode: one workgroup (4 wave64s) codes one 16 KiB chunk at a time,
+//   striding over the chunks in launch order.  Per chunk: 16-byte coalesced
+//   loads (4 slices per thread) -> per-value code in registers -> per-
+//   subchunk bit length by a half-wave max -> compressed/raw decision ->
+//   publish the size -> OR-pack the codes into an LDS staging buffer -> wave
+//   0 runs a two-level decoupled look-back for the chunk's stream offset ->
+//   realigned 16-byte stores to the stream.
+// decode: one workgroup of 1024 threads prefix-sums the uint16 size table
+//   into per-chunk offsets; then workgroups stride over the chunks with no
+//   cross-workgroup dependency: aligned 16-byte loads of the payload's cover
+//   into LDS -> per-lane bit extraction -> inverse transform -> 16-byte
+//   coalesced stores.
 // Reference counterparts: d_encode / d_decode, sleek_common_gpu.h:474-768
 // (lossless) and :989-1279 (lossy).
 #include "sleek_device.h"
@@ -90,13 +91,135 @@ __device__ __forceinline__ void store_slice(uint8_t* dst, int off, int osize, ui
     if (off + 4 * i + 4 <= osize) ((uint32_t*)(dst + off))[i] = d[i];
 }
 
+// Compressed size of chunk j, computed by one whole wave from the input (the
+// same bit lengths the chunk's own workgroup finds).  Used by a look-back
+// that has waited too long on an unpublished predecessor.
+template <typename W, bool LOSSY>
+__device__ uint32_t wave_chunk_size(const EncodeArgs& a, long long j)
+{
+  const int lane = threadIdx.x & 63;
+  const long long base = j * kChunk;
+  const int osize = (int)min((long long)kChunk, a.insize - base);
+  const uint8_t* const src = a.in + base;
+  uint32_t nb = 0;
+  for (int r = 0; r < kChunk / (64 * 16); r++) {   // 16 slices of 1 KiB: subchunks 2r and 2r+1
+    W v[Word<W>::PerSlice];
+    slice_values(load_slice(src, r * 1024 + lane * 16, osize), v);
+    W orv = 0;
+#pragma unroll
+    for (int k = 0; k < Word<W>::PerSlice; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, (W)a.offs);
+    const int bl = half_wave_max(bit_length(orv));
+    nb += (uint32_t)(__shfl(bl, 0, 64) + __shfl(bl, 32, 64));
+  }
+  const int newsize = kSub + 2 + (int)(nb * Word<W>::PerSub / 8);
+  return (uint32_t)(newsize < osize ? newsize : osize);
+}
+
+// Compressed size of group gj (sum over its chunks), by one whole wave:
+// published aggregate words are used, every other chunk is sized from the
+// input.
+template <typename W, bool LOSSY>
+__device__ uint64_t wave_group_size(const EncodeArgs& a, const ScanState& s, long long gj, long long nchunks)
+{
+  const int lane = threadIdx.x & 63;
+  const long long first = gj * kGroup;
+  const int members = (int)min((long long)kGroup, nchunks - first);
+  uint64_t f = (lane < members) ? flag_load(s.chunk + first + lane) : kAgg;
+  uint64_t mine = ((f >> 62) == 1) ? (f & kValMask) : 0;
+  uint64_t todo = __ballot(lane < members && (f >> 62) != 1);
+  while (todo) {
+    const int l = __ffsll((unsigned long long)todo) - 1;
+    const uint32_t sz = wave_chunk_size<W, LOSSY>(a, first + l);
+    if (lane == l) mine = sz;
+    todo &= todo - 1;
+  }
+  return wave_sum(mine);
+}
+
+// Help threshold of a look-back: after this long (100 MHz real-time ticks)
+// without progress, the wave sizes the blocking chunk or group itself.
+constexpr uint64_t kHelpTicks = 2000;   // 20 us
+
+// Two-level decoupled look-back, run by one whole wave; returns the sum of
+// the compressed sizes of chunks 0..c-1 in every lane.  Lanes watch the
+// preceding chunks of c's own group and, at the same time, a 64-group window
+// of earlier groups, so one round trip resolves up to 64 groups (4096
+// chunks) and the inclusive frontier advances that far per round trip.
+// Workgroups take chunks in launch order, so the words waited on are
+// normally published within a round trip or two; a word still missing after
+// kHelpTicks is computed from the input and published by the waiter, so
+// the look-back finishes even when a predecessor's workgroup is not running.
+template <typename W, bool LOSSY>
+__device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long c, long long nchunks)
+{
+  const int lane = threadIdx.x & 63;
+  const int q = (int)(c % kGroup);                 // chunks of the group before c
+  long long gend = c / kGroup;                      // group window: groups gend-64 .. gend-1
+  bool have_partial = (q == 0);
+  uint64_t partial = 0, gacc = 0;
+  uint64_t since = __builtin_amdgcn_s_memrealtime();
+  while (true) {
+    uint64_t f = kAgg;                              // lanes beyond the group: ready, value 0
+    if (!have_partial && lane < q) f = flag_load(s.chunk + (c - 1 - lane));
+    const long long gj = gend - 1 - lane;
+    const uint64_t fg = (gj >= 0) ? flag_load(s.group + gj) : kInc;   // before group 0: inclusive 0
+    if (!have_partial) {
+      const uint64_t inc = __ballot((f >> 62) == 2);
+      const uint64_t rdy = __ballot((f >> 62) != 0);
+      if (inc) {
+        const int p = __ffsll((unsigned long long)inc) - 1;
+        const uint64_t need = (2ull << p) - 1;      // p < 63 here
+        if ((rdy & need) == need) return wave_sum(lane <= p ? (f & kValMask) : 0);
+      } else if (rdy == ~0ull) {
+        partial = wave_sum(f & kValMask);
+        have_partial = true;
+        since = __builtin_amdgcn_s_memrealtime();
+      }
+    }
+    if (have_partial) {
+      const uint64_t inc = __ballot((fg >> 62) == 2);
+      const uint64_t rdy = __ballot((fg >> 62) != 0);
+      if (inc) {
+        const int p = __ffsll((unsigned long long)inc) - 1;
+        const uint64_t need = (p == 63) ? ~0ull : ((2ull << p) - 1);
+        if ((rdy & need) == need) return partial + gacc + wave_sum(lane <= p ? (fg & kValMask) : 0);
+      } else if (rdy == ~0ull) {
+        gacc += wave_sum(fg & kValMask);
+        gend -= 64;
+        since = __builtin_amdgcn_s_memrealtime();
+        continue;
+      }
+    }
+    if (__builtin_amdgcn_s_memrealtime() - since > kHelpTicks) {
+      // size the nearest unpublished word ourselves and publish it
+      if (!have_partial) {
+        const uint64_t miss = __ballot(lane < q && (f >> 62) == 0);
+        if (miss) {
+          const long long jc = c - __ffsll((unsigned long long)miss);
+          const uint32_t sz = wave_chunk_size<W, LOSSY>(a, jc);
+          if (lane == 0) flag_max(s.chunk + jc, kAgg | sz);
+        }
+      } else {
+        const uint64_t miss = __ballot((fg >> 62) == 0);
+        if (miss) {
+          const long long gm = gend - __ffsll((unsigned long long)miss);
+          const uint64_t sz = wave_group_size<W, LOSSY>(a, s, gm, nchunks);
+          if (lane == 0) flag_max(s.group + gm, (gm == 0 ? kInc : kAgg) | sz);
+        }
+      }
+      since = __builtin_amdgcn_s_memrealtime();
+      continue;
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
 template <typename W, bool LOSSY, int WAVES>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void encode_kernel(EncodeArgs a)
 {
   constexpr int PS = Word<W>::PerSlice;
   __shared__ uint32_t stage[kStageBytes / 4];
   __shared__ uint32_t lgs[8];
-  __shared__ long long s_ticket[2];
   __shared__ unsigned long long s_excl;
 
   const int t = threadIdx.x;
@@ -105,7 +228,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
   const long long data_base = 8 + 2 * nchunks;
   uint16_t* const sizes = (uint16_t*)(a.out + 8);
-  uint64_t* const flags = a.ws + kWsFlags;
+  const ScanState st = scan_state(a.ws, nchunks);
   const W offs = (W)a.offs;
 
   if (nchunks == 0) {
@@ -116,16 +239,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     return;
   }
 
-  // Tickets run one chunk ahead: the next one is taken while the current
-  // chunk is coded, so the atomic's latency is off the critical path.  A
-  // block only ever holds tickets above the one it is coding, so the
-  // smallest unfinished chunk always belongs to a block that is coding it.
-  if (t == 0) s_ticket[0] = (long long)take_ticket(a.ws);
-  for (int it = 0;; it++) {
+  // Workgroups stride over the chunks in launch order; no ticket counter
+  // (a single same-address atomic per chunk serialises at ~12 ns each).
+  for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) {
     for (int i = t; i < kChunk / 16 + 1; i += kThreads) ((uint4*)stage)[i] = make_uint4(0u, 0u, 0u, 0u);
-    __syncthreads();
-    const long long c = uni(s_ticket[it & 1]);
-    if (c >= nchunks) break;
     const long long base = c * kChunk;
     const int osize = (int)min((long long)kChunk, a.insize - base);
     const uint8_t* const src = a.in + base;
@@ -135,8 +252,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     uint4 raw[kRounds];
 #pragma unroll
     for (int r = 0; r < kRounds; r++) raw[r] = load_slice(src, r * kThreads * 16 + t * 16, osize);
-    long long next = 0;
-    if (t == 0) next = (long long)take_ticket(a.ws);
 
 #pragma unroll
     for (int r = 0; r < kRounds; r++) {
@@ -165,8 +280,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 
     if (t == 0) {
       // publish the chunk's size at once: successors can sum it while this
-      // block packs and looks back
-      flag_store(flags + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
+      // workgroup packs and looks back
+      publish_size(st, c, nchunks, (uint32_t)csize);
       sizes[c] = (uint16_t)csize;
     }
 
@@ -195,11 +310,10 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 
     if (t < 64) {
       // wave 0, after its share of the packing: the chunk's stream offset
-      const uint64_t excl = (c == 0) ? 0ull : look_back(flags, c);
+      const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
       if (t == 0) {
-        if (c != 0) flag_store(flags + c, kInc | (excl + (uint64_t)csize));
+        if (c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)csize);
         s_excl = excl;
-        s_ticket[(it + 1) & 1] = next;
       }
     }
     __syncthreads();
@@ -225,9 +339,12 @@ __device__ __forceinline__ bool header_ok(const DecodeArgs& a, long long hdr, lo
 }
 
 // Exclusive prefix sums of the uint16 size table into the workspace (one
-// int64 per chunk plus the total), by one workgroup of 1024 threads sweeping
-// the table in tiles of 8192 entries.  Also writes *decsize.
+// int64 per chunk plus the total), by one workgroup of 1024 threads: each
+// thread takes 32 consecutive entries (8-byte loads), so one pass covers
+// 32768 chunks (512 MiB of f32 input) and larger tables loop.  Also writes
+// *decsize.
 constexpr int kScanThreads = 1024;
+constexpr int kScanPer = 32;
 __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
 {
   __shared__ long long wsum[kScanThreads / 64];
@@ -240,19 +357,30 @@ __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
     return;
   }
   if (t == 0) *a.decsize = hdr;
-  const uint8_t* const tab = a.in + 8;
+  const uint16_t* const tab = (const uint16_t*)(a.in + 8);
   long long* const offs = (long long*)(a.ws + kWsFlags);
   long long carry = 0;
-  for (long long base = 0; base < nchunks; base += (long long)kScanThreads * 8) {
-    const long long i0 = base + (long long)t * 8;
-    uint32_t v[8];
+  for (long long base = 0; base < nchunks; base += (long long)kScanThreads * kScanPer) {
+    const long long i0 = base + (long long)t * kScanPer;
+    uint32_t v[kScanPer];
+    if (i0 + kScanPer <= nchunks) {
+      const unsigned long long* p = (const unsigned long long*)(tab + i0);   // 8-byte aligned
 #pragma unroll
-    for (int k = 0; k < 8; k++) v[k] = (i0 + k < nchunks) ? (uint32_t)*(const uint16_t*)(tab + 2 * (i0 + k)) : 0u;
+      for (int k = 0; k < kScanPer / 4; k++) {
+        const unsigned long long w = p[k];
+        v[4 * k] = (uint32_t)(w & 0xffff);
+        v[4 * k + 1] = (uint32_t)((w >> 16) & 0xffff);
+        v[4 * k + 2] = (uint32_t)((w >> 32) & 0xffff);
+        v[4 * k + 3] = (uint32_t)(w >> 48);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanPer; k++) v[k] = (i0 + k < nchunks) ? (uint32_t)tab[i0 + k] : 0u;
+    }
     uint32_t local = 0;
 #pragma unroll
-    for (int k = 0; k < 8; k++) local += v[k];
-    // inclusive wave scan
-    uint32_t inc = local;
+    for (int k = 0; k < kScanPer; k++) local += v[k];
+    uint32_t inc = local;   // inclusive wave scan of the per-thread sums
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
       const uint32_t y = __shfl_up(inc, d, 64);
@@ -268,10 +396,21 @@ __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
       tile += x;
     }
     long long run = carry + before + (long long)(inc - local);
+    if (i0 + kScanPer <= nchunks) {
+      longlong2* o = (longlong2*)(offs + i0);   // 16-byte aligned: i0 is a multiple of 32
 #pragma unroll
-    for (int k = 0; k < 8; k++) {
-      if (i0 + k < nchunks) offs[i0 + k] = run;
-      run += v[k];
+      for (int k = 0; k < kScanPer / 2; k++) {
+        const long long r0 = run;
+        run += v[2 * k];
+        o[k] = make_longlong2(r0, run);
+        run += v[2 * k + 1];
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < kScanPer; k++) {
+        if (i0 + k < nchunks) offs[i0 + k] = run;
+        run += v[k];
+      }
     }
     carry += tile;
     __syncthreads();

@@ -58,6 +58,6 @@ def test_barriers_in_single_outer_loop(asm):
     dec = [k for k in b if "decode_kernel" in k]
     assert len(enc) == 8 and len(dec) == 4
     for k in enc:
-        assert b[k] == [1, 1, 1, 1], (k, b[k])   # tickets, bit lengths, payload ready, stage reuse
+        assert b[k] == [1, 1, 1], (k, b[k])      # bit lengths, payload ready, stage reuse
     for k in dec:
         assert b[k] == [1, 1], (k, b[k])          # payload cover loaded, stage reuse

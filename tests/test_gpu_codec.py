@@ -170,3 +170,27 @@ def test_corrupt_stream_detected(gpu):
     d, _ = codec.decompress(s, torch.float32, numel=a.size, workspace=ws)
     assert ws.status() == 0
     assert d.cpu().numpy().tobytes() == a.tobytes()
+
+
+@pytest.mark.parametrize("dt,n,eb", [("f32", 100_003, 0.0), ("f32", 100_003, 1e-3), ("f64", 50_001, 0.0),
+                                     ("f64", 50_001, 1e-6), ("f32", 0, 0.0)])
+def test_cpp_dropin_headers(gpu, tmp_path, dt, n, eb):
+    """A C++ caller of the reference API (drop-in headers, g++-built) gets the oracle's bytes."""
+    import subprocess
+
+    say(f"drop-in C++ {dt} n={n} eb={eb}")
+    exe = os.path.join(os.path.dirname(__file__), "cpp", "dropin_check")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp/dropin_check not built")
+    so, do = tmp_path / "s.bin", tmp_path / "d.bin"
+    r = subprocess.run([exe, dt, str(n), str(eb), str(so), str(do)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    npdt = np.float32 if dt == "f32" else np.float64
+    i = np.arange(n, dtype=np.int64)
+    a = (i.astype(np.float64) * 0.001 + ((i.astype(np.uint64) * np.uint64(2654435761)) % np.uint64(1000))
+         .astype(np.float64) * 1e-6).astype(npdt)
+    ebv = None if eb == 0 else float(npdt(eb))
+    ref = po.compress(a, ebv)
+    assert open(so, "rb").read() == ref.tobytes()
+    od, rc = po.decompress(ref, npdt, ebv)
+    assert rc == 0 and open(do, "rb").read() == od.tobytes()

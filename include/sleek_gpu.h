@@ -102,8 +102,8 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
                                long long capacity, long long* d_decsize, void* d_workspace,
                                long long workspace_bytes, int blocks, void* stream);
 /* Launch-shape knobs, found by the autotuner (sleek_amd/autotune.py); no
- * reference counterpart.  encode_waves: 8 = register budget for 8 waves per
- * SIMD, any other positive value = compiler's choice; encode_grid /
+ * reference counterpart.  encode_waves: >= 4 = register budget for 4 waves
+ * per SIMD, 1..3 = compiler's choice (0 = default, 4); encode_grid /
  * decode_grid: grid sizes (0 = automatic).  They change speed only, never
  * the bytes.  Applies to calls made after it returns. */
 int sleek_gpu_set_tuning(int encode_waves, int encode_grid, int decode_grid);

@@ -37,7 +37,7 @@ long long ws_bytes_for(long long insize)
 }
 
 // tuning knobs (sleek_gpu_set_tuning); none changes the bytes produced
-int g_encode_waves = 8;
+int g_encode_waves = 4;
 int g_encode_grid = 0;
 int g_decode_grid = 0;
 
@@ -309,7 +309,7 @@ int sleek_gpu_set_tuning(int encode_waves, int encode_grid, int decode_grid)
 {
   if (encode_waves < 0 || encode_grid < 0 || decode_grid < 0) return SLEEK_EINVAL;
   std::lock_guard<std::mutex> lk(g_mu);
-  g_encode_waves = encode_waves > 0 ? encode_waves : 8;
+  g_encode_waves = encode_waves > 0 ? encode_waves : 4;
   g_encode_grid = encode_grid;
   g_decode_grid = decode_grid;
   return SLEEK_OK;

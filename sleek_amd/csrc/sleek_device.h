@@ -15,7 +15,12 @@ constexpr int kSub = 32;                 // subchunks per chunk
 constexpr int kSubBytes = kChunk / kSub; // 512 bytes per subchunk
 
 // ---- launch geometry ----
+// A workgroup of 4 waves codes one chunk; wave w owns subchunks 8w..8w+7,
+// one per round, and each lane holds one 8-byte slice of the subchunk, so
+// every per-subchunk quantity (bit length, bit offset) is wave-uniform.
 constexpr int kThreads = 256;                       // 4 waves of 64
+constexpr int kWaveRounds = kSub / (kThreads / 64); // subchunks per wave
+static_assert(kWaveRounds % 4 == 0, "a wave's bit lengths fill whole dwords");
 constexpr int kRounds = kChunk / (kThreads * 16);   // 16-byte slices per thread and chunk
 constexpr int kStageBytes = kChunk + 64;            // LDS staging buffer (payload + realignment slack)
 
@@ -43,11 +48,13 @@ template <typename W> struct Word;
 template <> struct Word<uint32_t> {
   static constexpr int TB = 32, EXPB = 8, MANB = 23;
   static constexpr int PerSlice = 4;    // values per 16-byte slice
+  static constexpr int PerLane = 2;     // values per 8-byte slice (one lane's share of a subchunk)
   static constexpr int PerSub = 128;    // values per subchunk
 };
 template <> struct Word<uint64_t> {
   static constexpr int TB = 64, EXPB = 11, MANB = 52;
   static constexpr int PerSlice = 2;
+  static constexpr int PerLane = 1;
   static constexpr int PerSub = 64;
 };
 
@@ -160,6 +167,39 @@ __device__ __forceinline__ uint4 values_slice(const uint64_t v[2])
   return make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
 }
 
+__device__ __forceinline__ void lane_values(const uint2& q, uint32_t v[2]) { v[0] = q.x; v[1] = q.y; }
+__device__ __forceinline__ void lane_values(const uint2& q, uint64_t v[1]) { v[0] = (uint64_t)q.x | ((uint64_t)q.y << 32); }
+__device__ __forceinline__ uint2 values_lane(const uint32_t v[2]) { return make_uint2(v[0], v[1]); }
+__device__ __forceinline__ uint2 values_lane(const uint64_t v[1]) { return make_uint2((uint32_t)v[0], (uint32_t)(v[0] >> 32)); }
+
+// A lane's codes as one bit field: code k at bit k*ln (ln <= word width,
+// so two float32 codes fit in 64 bits).
+__device__ __forceinline__ uint64_t lane_field(const uint32_t v[2], int ln) { return (uint64_t)v[0] | ((uint64_t)v[1] << ln); }
+__device__ __forceinline__ uint64_t lane_field(const uint64_t v[1], int) { return v[0]; }
+
+// OR over the 64 lanes of a wave, returned as a wave-uniform value: DPP
+// swaps inside rows of 16 lanes, then the four row results are combined.
+__device__ __forceinline__ uint32_t wave_or(uint32_t v)
+{
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x141, 0xF, 0xF, false);  // row_half_mirror
+  v |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x140, 0xF, 0xF, false);  // row_mirror
+  return (uint32_t)(__builtin_amdgcn_readlane((int)v, 0) | __builtin_amdgcn_readlane((int)v, 16) |
+                    __builtin_amdgcn_readlane((int)v, 32) | __builtin_amdgcn_readlane((int)v, 48));
+}
+
+// Bit length of the largest code over a wave (the bit length of the OR of
+// the codes), wave-uniform.
+__device__ __forceinline__ int wave_bit_length(uint32_t orv) { const uint32_t o = wave_or(orv); return o ? 32 - __clz(o) : 0; }
+__device__ __forceinline__ int wave_bit_length(uint64_t orv)
+{
+  const uint32_t hi = wave_or((uint32_t)(orv >> 32));
+  if (hi) return 64 - __clz(hi);
+  const uint32_t lo = wave_or((uint32_t)orv);
+  return lo ? 32 - __clz(lo) : 0;
+}
+
 __device__ __forceinline__ int bit_length(uint32_t x) { return 32 - __clz(x); }
 __device__ __forceinline__ int bit_length(uint64_t x) { return 64 - __clzll(x); }
 
@@ -201,6 +241,13 @@ __device__ __forceinline__ void reissue(uint4& q)
 {
   asm volatile("" : "+v"(q.x), "+v"(q.y), "+v"(q.z), "+v"(q.w));
 }
+__device__ __forceinline__ void reissue(uint32_t& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void reissue(uint64_t& x)
+{
+  uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+  asm volatile("" : "+v"(lo), "+v"(hi));
+  x = (uint64_t)lo | ((uint64_t)hi << 32);
+}
 
 // Any byte of x above n (n <= 127), without carries between bytes.
 __device__ __forceinline__ bool any_byte_above(uint32_t x, uint32_t n)
@@ -231,6 +278,28 @@ __device__ __forceinline__ int sub_bit_offset(const uint32_t lg[8], int s)
 }
 
 __device__ __forceinline__ int sub_length(const uint32_t lg[8], int s) { return (int)((lg[s >> 2] >> (8 * (s & 3))) & 0xffu); }
+
+// OR a bit field of up to 64 bits into a zeroed LDS bit buffer at bit p
+// (the field spans at most three dwords).
+__device__ __forceinline__ void or_field(uint32_t* st, int p, uint64_t f)
+{
+  const int d = p >> 5;
+  const uint32_t sh = (uint32_t)p & 31u;
+  const uint64_t lo = f << sh;
+  const uint32_t w2 = (uint32_t)(((f >> 32) << sh) >> 32);
+  atomicOr(&st[d], (uint32_t)lo);
+  atomicOr(&st[d + 1], (uint32_t)(lo >> 32));
+  if (w2) atomicOr(&st[d + 2], w2);
+}
+
+// The 64 bits of an LDS bit buffer starting at bit p.
+__device__ __forceinline__ uint64_t read_field(const uint32_t* st, int p)
+{
+  const int d = p >> 5;
+  const uint32_t sh = (uint32_t)p & 31u;
+  const uint32_t w0 = st[d], w1 = st[d + 1], w2 = st[d + 2];
+  return (uint64_t)__builtin_amdgcn_alignbit(w1, w0, sh) | ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, sh) << 32);
+}
 
 // ---- scan state ----
 __device__ __forceinline__ uint64_t flag_load(const uint64_t* p)

@@ -27,8 +27,9 @@ struct DecodeArgs {
   unsigned long long offs;
 };
 
-// waves: occupancy target of the encoder (8 = register budget of 8 waves per
-// SIMD; anything lower = the compiler's choice).  Output bytes never depend
+// waves: occupancy target of the encoder (>= 4: register budget of 4 waves
+// per SIMD, the most its two LDS staging buffers allow; lower: the
+// compiler's choice).  Output bytes never depend
 // on it or on the grid size.
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int waves, hipStream_t stream);
 hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream);

@@ -19,50 +19,6 @@
 
 namespace sleek {
 
-template <typename W>
-__device__ __forceinline__ void or_code(uint32_t* st, int b, W v);
-
-template <>
-__device__ __forceinline__ void or_code<uint32_t>(uint32_t* st, int b, uint32_t v)
-{
-  const int d = b >> 5;
-  const uint32_t sh = (uint32_t)b & 31u;
-  atomicOr(&st[d], v << sh);
-  if (sh) {
-    const uint32_t hi = v >> (32u - sh);
-    if (hi) atomicOr(&st[d + 1], hi);
-  }
-}
-
-template <>
-__device__ __forceinline__ void or_code<uint64_t>(uint32_t* st, int b, uint64_t v)
-{
-  const int d = b >> 5;
-  const uint32_t sh = (uint32_t)b & 31u;
-  const uint64_t lo = v << sh;
-  const uint32_t w0 = (uint32_t)lo, w1 = (uint32_t)(lo >> 32);
-  const uint32_t w2 = sh ? (uint32_t)(v >> (64u - sh)) : 0u;
-  if (w0) atomicOr(&st[d], w0);
-  if (w1) atomicOr(&st[d + 1], w1);
-  if (w2) atomicOr(&st[d + 2], w2);
-}
-
-__device__ __forceinline__ uint32_t extract_code(const uint32_t* st, int b, int ln, uint32_t*)
-{
-  const int d = b >> 5;
-  const uint32_t v = __builtin_amdgcn_alignbit(st[d + 1], st[d], (uint32_t)b & 31u);
-  return ln >= 32 ? v : (v & ((1u << ln) - 1u));
-}
-
-__device__ __forceinline__ uint64_t extract_code(const uint32_t* st, int b, int ln, uint64_t*)
-{
-  const int d = b >> 5;
-  const uint32_t sh = (uint32_t)b & 31u;
-  const uint32_t w0 = st[d], w1 = st[d + 1], w2 = st[d + 2];
-  const uint64_t v = (uint64_t)__builtin_amdgcn_alignbit(w1, w0, sh) | ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, sh) << 32);
-  return ln >= 64 ? v : (v & ((1ull << ln) - 1ull));
-}
-
 // Load the 16-byte slice at byte off of a chunk of osize bytes, zero beyond
 // osize.  The slice is 16-byte aligned, so when any of its bytes is valid the
 // whole slice lies in a mapped page and can be loaded at once.
@@ -91,6 +47,22 @@ __device__ __forceinline__ void store_slice(uint8_t* dst, int off, int osize, ui
     if (off + 4 * i + 4 <= osize) ((uint32_t*)(dst + off))[i] = d[i];
 }
 
+// The 8-byte slice at byte off of a chunk of osize bytes (osize is a
+// multiple of 4), zero beyond osize; aligned, so it never crosses a page.
+__device__ __forceinline__ uint2 load_lane(const uint8_t* src, int off, int osize)
+{
+  if (off >= osize) return make_uint2(0u, 0u);
+  uint2 q = *(const uint2*)(src + off);
+  if (osize - off < 8) q.y = 0u;
+  return q;
+}
+
+__device__ __forceinline__ void store_lane(uint8_t* dst, int off, int osize, uint2 q)
+{
+  if (off + 8 <= osize) *(uint2*)(dst + off) = q;
+  else if (off < osize) *(uint32_t*)(dst + off) = q.x;
+}
+
 // Compressed size of chunk j, computed by one whole wave from the input (the
 // same bit lengths the chunk's own workgroup finds).  Used by a look-back
 // that has waited too long on an unpublished predecessor.
@@ -102,14 +74,13 @@ __device__ uint32_t wave_chunk_size(const EncodeArgs& a, long long j)
   const int osize = (int)min((long long)kChunk, a.insize - base);
   const uint8_t* const src = a.in + base;
   uint32_t nb = 0;
-  for (int r = 0; r < kChunk / (64 * 16); r++) {   // 16 slices of 1 KiB: subchunks 2r and 2r+1
-    W v[Word<W>::PerSlice];
-    slice_values(load_slice(src, r * 1024 + lane * 16, osize), v);
+  for (int s = 0; s < kSub; s++) {
+    W v[Word<W>::PerLane];
+    lane_values(load_lane(src, s * kSubBytes + lane * 8, osize), v);
     W orv = 0;
 #pragma unroll
-    for (int k = 0; k < Word<W>::PerSlice; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, (W)a.offs);
-    const int bl = half_wave_max(bit_length(orv));
-    nb += (uint32_t)(__shfl(bl, 0, 64) + __shfl(bl, 32, 64));
+    for (int k = 0; k < Word<W>::PerLane; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, (W)a.offs);
+    nb += (uint32_t)wave_bit_length(orv);
   }
   const int newsize = kSub + 2 + (int)(nb * Word<W>::PerSub / 8);
   return (uint32_t)(newsize < osize ? newsize : osize);
@@ -138,15 +109,18 @@ __device__ uint64_t wave_group_size(const EncodeArgs& a, const ScanState& s, lon
 
 // Help threshold of a look-back: after this long (100 MHz real-time ticks)
 // without progress, the wave sizes the blocking chunk or group itself.
-constexpr uint64_t kHelpTicks = 2000;   // 20 us
+constexpr uint64_t kHelpTicks = 10000;   // 100 us
 
 // Two-level decoupled look-back, run by one whole wave; returns the sum of
-// the compressed sizes of chunks 0..c-1 in every lane.  Lanes watch the
-// preceding chunks of c's own group and, at the same time, a 64-group window
-// of earlier groups, so one round trip resolves up to 64 groups (4096
-// chunks) and the inclusive frontier advances that far per round trip.
+// the compressed sizes of chunks 0..c-1 in every lane.  First the preceding
+// chunks of c's own group are watched (one lane each); once they are all
+// published, a window of earlier groups is watched, starting 4 groups wide
+// and doubling to 64 as it slides back, so the inclusive frontier can
+// advance up to 4096 chunks per round trip while a poll reads few words.
+// Polls back off exponentially: the words live at device scope, and many
+// waiting waves polling at full rate would compete with the data traffic.
 // Workgroups take chunks in launch order, so the words waited on are
-// normally published within a round trip or two; a word still missing after
+// normally published within a few round trips; a word still missing after
 // kHelpTicks is computed from the input and published by the waiter, so
 // the look-back finishes even when a predecessor's workgroup is not running.
 template <typename W, bool LOSSY>
@@ -154,16 +128,16 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
 {
   const int lane = threadIdx.x & 63;
   const int q = (int)(c % kGroup);                 // chunks of the group before c
-  long long gend = c / kGroup;                      // group window: groups gend-64 .. gend-1
+  long long gend = c / kGroup;                      // group window: groups gend-gwin .. gend-1
+  int gwin = 4;
   bool have_partial = (q == 0);
   uint64_t partial = 0, gacc = 0;
   uint64_t since = __builtin_amdgcn_s_memrealtime();
+  int naps = 1;
   while (true) {
-    uint64_t f = kAgg;                              // lanes beyond the group: ready, value 0
-    if (!have_partial && lane < q) f = flag_load(s.chunk + (c - 1 - lane));
-    const long long gj = gend - 1 - lane;
-    const uint64_t fg = (gj >= 0) ? flag_load(s.group + gj) : kInc;   // before group 0: inclusive 0
+    bool progressed = false;
     if (!have_partial) {
+      const uint64_t f = (lane < q) ? flag_load(s.chunk + (c - 1 - lane)) : kAgg;   // beyond the group: 0
       const uint64_t inc = __ballot((f >> 62) == 2);
       const uint64_t rdy = __ballot((f >> 62) != 0);
       if (inc) {
@@ -173,63 +147,80 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
       } else if (rdy == ~0ull) {
         partial = wave_sum(f & kValMask);
         have_partial = true;
-        since = __builtin_amdgcn_s_memrealtime();
+        progressed = true;
       }
-    }
-    if (have_partial) {
-      const uint64_t inc = __ballot((fg >> 62) == 2);
-      const uint64_t rdy = __ballot((fg >> 62) != 0);
-      if (inc) {
-        const int p = __ffsll((unsigned long long)inc) - 1;
-        const uint64_t need = (p == 63) ? ~0ull : ((2ull << p) - 1);
-        if ((rdy & need) == need) return partial + gacc + wave_sum(lane <= p ? (fg & kValMask) : 0);
-      } else if (rdy == ~0ull) {
-        gacc += wave_sum(fg & kValMask);
-        gend -= 64;
-        since = __builtin_amdgcn_s_memrealtime();
-        continue;
-      }
-    }
-    if (__builtin_amdgcn_s_memrealtime() - since > kHelpTicks) {
-      // size the nearest unpublished word ourselves and publish it
-      if (!have_partial) {
+      if (!progressed && __builtin_amdgcn_s_memrealtime() - since > kHelpTicks) {
         const uint64_t miss = __ballot(lane < q && (f >> 62) == 0);
         if (miss) {
           const long long jc = c - __ffsll((unsigned long long)miss);
           const uint32_t sz = wave_chunk_size<W, LOSSY>(a, jc);
           if (lane == 0) flag_max(s.chunk + jc, kAgg | sz);
         }
-      } else {
+        progressed = true;
+      }
+    } else {
+      const long long gj = gend - 1 - lane;
+      const uint64_t fg = (lane >= gwin) ? kAgg : ((gj >= 0) ? flag_load(s.group + gj) : kInc);  // before group 0: inclusive 0
+      const uint64_t inc = __ballot((fg >> 62) == 2);
+      const uint64_t rdy = __ballot((fg >> 62) != 0);
+      if (inc) {
+        const int p = __ffsll((unsigned long long)inc) - 1;
+        const uint64_t need = (2ull << p) - 1;      // p < gwin <= 64; lanes >= gwin carry 0
+        if ((rdy & need) == need) return partial + gacc + wave_sum(lane <= p ? (fg & kValMask) : 0);
+      } else if (rdy == ~0ull) {
+        gacc += wave_sum(fg & kValMask);
+        gend -= gwin;
+        if (gwin < 64) gwin *= 2;
+        progressed = true;
+      }
+      if (!progressed && __builtin_amdgcn_s_memrealtime() - since > kHelpTicks) {
         const uint64_t miss = __ballot((fg >> 62) == 0);
         if (miss) {
           const long long gm = gend - __ffsll((unsigned long long)miss);
           const uint64_t sz = wave_group_size<W, LOSSY>(a, s, gm, nchunks);
           if (lane == 0) flag_max(s.group + gm, (gm == 0 ? kInc : kAgg) | sz);
         }
+        progressed = true;
       }
+    }
+    if (progressed) {
       since = __builtin_amdgcn_s_memrealtime();
+      naps = 1;
       continue;
     }
-    __builtin_amdgcn_s_sleep(1);
+    for (int i = 0; i < naps; i++) __builtin_amdgcn_s_sleep(2);
+    if (naps < 32) naps *= 2;
   }
 }
 
+// Encoder: each workgroup walks its chunks (launch order, stride = grid) as
+// a three-stage pipeline, one chunk per stage per iteration:
+//   iteration k:  slices of chunk k+1 are loaded into registers (prefetch),
+//                 chunk k is coded, its size published and its payload
+//                 packed into LDS buffer k&1,
+//                 chunk k-1's look-back runs (wave 0) and its payload, waiting
+//                 in buffer (k-1)&1 since iteration k-1, is stored.
+// Deferring the look-back by one iteration gives the chunk's predecessors a
+// whole iteration to publish their sizes, so it rarely waits; the prefetch
+// hides the load latency.  Two staging buffers put the occupancy at four
+// workgroups per CU, which leaves room for the prefetch registers.
 template <typename W, bool LOSSY, int WAVES>
-__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 8))) void encode_kernel(EncodeArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 4))) void encode_kernel(EncodeArgs a)
 {
-  constexpr int PS = Word<W>::PerSlice;
-  __shared__ uint32_t stage[kStageBytes / 4];
+  constexpr int PL = Word<W>::PerLane;
+  __shared__ uint32_t stage[2][kStageBytes / 4];
   __shared__ uint32_t lgs[8];
   __shared__ unsigned long long s_excl;
 
   const int t = threadIdx.x;
-  const int lane32 = t & 31;
-  const int half = t >> 5;
+  const int lane = t & 63;
+  const int wv = t >> 6;
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
   const long long data_base = 8 + 2 * nchunks;
   uint16_t* const sizes = (uint16_t*)(a.out + 8);
   const ScanState st = scan_state(a.ws, nchunks);
   const W offs = (W)a.offs;
+  const int woff = wv * (kWaveRounds * kSubBytes) + lane * 8;   // this lane's byte in round 0
 
   if (nchunks == 0) {
     if (blockIdx.x == 0 && t == 0) {
@@ -239,92 +230,143 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
     return;
   }
 
-  // Workgroups stride over the chunks in launch order; no ticket counter
-  // (a single same-address atomic per chunk serialises at ~12 ns each).
-  for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    for (int i = t; i < kChunk / 16 + 1; i += kThreads) ((uint4*)stage)[i] = make_uint4(0u, 0u, 0u, 0u);
-    const long long base = c * kChunk;
-    const int osize = (int)min((long long)kChunk, a.insize - base);
-    const uint8_t* const src = a.in + base;
+  // prefetched slices of the chunk coded next
+  uint2 nxt[kWaveRounds];
+  long long c = blockIdx.x;
+  if (c < nchunks) {
+    const int os = (int)min((long long)kChunk, a.insize - c * kChunk);
+#pragma unroll
+    for (int r = 0; r < kWaveRounds; r++) nxt[r] = load_lane(a.in + c * kChunk, woff + r * kSubBytes, os);
+  }
+  long long prev = -1;     // chunk whose payload waits in stage[pb] (wave-uniform)
+  int prev_csize = 0, pb = 0, cb = 0;
 
-    // Only the raw slices stay live across the barriers (16 VGPRs); the
-    // codes are cheap and are computed again in the packing pass.
-    uint4 raw[kRounds];
-#pragma unroll
-    for (int r = 0; r < kRounds; r++) raw[r] = load_slice(src, r * kThreads * 16 + t * 16, osize);
+  for (;; c += gridDim.x) {
+    const bool have = c < nchunks;
+    int csize = 0;
+    if (have) {
+      uint32_t* const sb = stage[cb];
+      for (int i = t; i < kChunk / 16 + 1; i += kThreads) ((uint4*)sb)[i] = make_uint4(0u, 0u, 0u, 0u);
+      const long long base = c * kChunk;
+      const int osize = (int)min((long long)kChunk, a.insize - base);
 
+      W code[kWaveRounds][PL];
 #pragma unroll
-    for (int r = 0; r < kRounds; r++) {
-      W v[PS];
-      slice_values(raw[r], v);
-      W orv = 0;
+      for (int r = 0; r < kWaveRounds; r++) {
+        W v[PL];
+        lane_values(nxt[r], v);
 #pragma unroll
-      for (int k = 0; k < PS; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
-      const int bl = half_wave_max(bit_length(orv));
-      if (lane32 == 0) ((uint8_t*)lgs)[r * 8 + half] = (uint8_t)bl;
-    }
-    __syncthreads();
+        for (int k = 0; k < PL; k++) code[r][k] = v[k];
+      }
+      // prefetch the next chunk of this workgroup
+      const long long cn = c + gridDim.x;
+      if (cn < nchunks) {
+        const int os = (int)min((long long)kChunk, a.insize - cn * kChunk);
+#pragma unroll
+        for (int r = 0; r < kWaveRounds; r++) nxt[r] = load_lane(a.in + cn * kChunk, woff + r * kSubBytes, os);
+      }
 
+      uint32_t lgw[kWaveRounds / 4];   // the wave's bit lengths, byte-packed (wave-uniform)
 #pragma unroll
-    for (int r = 0; r < kRounds; r++) reissue(raw[r]);
-    uint32_t L[8];
+      for (int i = 0; i < kWaveRounds / 4; i++) lgw[i] = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) L[i] = uni(lgs[i]);
-    uint32_t nb = 0;
+      for (int r = 0; r < kWaveRounds; r++) {
+        W orv = 0;
 #pragma unroll
-    for (int i = 0; i < 8; i++) nb += byte_sum(L[i]);
-    const int total_bits = (int)nb * Word<W>::PerSub;
-    const int newsize = kSub + 2 + total_bits / 8;
-    const bool packed = newsize < osize;   // osize <= kChunk, so also newsize < kChunk
-    const int csize = packed ? newsize : osize;
+        for (int k = 0; k < PL; k++) {
+          code[r][k] = to_code<W, LOSSY>(code[r][k], a.eb_e, a.thr_e, offs);
+          orv |= code[r][k];
+        }
+        lgw[r / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (r % 4));
+      }
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < kWaveRounds / 4; i++) lgs[wv * (kWaveRounds / 4) + i] = lgw[i];
+      }
+      __syncthreads();
 
-    if (t == 0) {
-      // publish the chunk's size at once: successors can sum it while this
-      // workgroup packs and looks back
-      publish_size(st, c, nchunks, (uint32_t)csize);
-      sizes[c] = (uint16_t)csize;
-    }
+      uint32_t L[8];
+#pragma unroll
+      for (int i = 0; i < 8; i++) L[i] = uni(lgs[i]);
+      uint32_t nb = 0;
+#pragma unroll
+      for (int i = 0; i < 8; i++) nb += byte_sum(L[i]);
+      const int total_bits = (int)nb * Word<W>::PerSub;
+      const int newsize = kSub + 2 + total_bits / 8;
+      const bool packed = newsize < osize;   // osize <= kChunk, so also newsize < kChunk
+      csize = packed ? newsize : osize;
 
-    if (packed) {
-      if (t < 8) stage[t] = L[t];                                // 32 bit-length bytes
-      if (t == 64) stage[8 + total_bits / 32] = (uint32_t)osize;  // int16 trailer
+      // publish the chunk's size at once; the group counter's reply is
+      // looked at after the packing
+      unsigned long long grp_old = 0;
+      if (t == 0) {
+        flag_max(st.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
+        grp_old = __hip_atomic_fetch_add(st.count + c / kGroup, (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
+                                         __HIP_MEMORY_SCOPE_AGENT);
+        sizes[c] = (uint16_t)csize;
+      }
+
+      if (packed) {
+        if (t < 8) sb[t] = L[t];                                // 32 bit-length bytes
+        if (t == 64) sb[8 + total_bits / 32] = (uint32_t)osize;  // int16 trailer
+        // wave-uniform subchunk index, bit length and bit offset per round
+        int p0 = 8 * kSub + sub_bit_offset<W>(L, wv * kWaveRounds);
 #pragma unroll
-      for (int r = 0; r < kRounds; r++) {
-        const int s = r * 8 + half;
-        const int ln = sub_length(L, s);
-        if (ln == 0) continue;
-        const int p0 = 8 * kSub + sub_bit_offset<W>(L, s) + lane32 * PS * ln;
-        W v[PS];
-        slice_values(raw[r], v);
+        for (int r = 0; r < kWaveRounds; r++) {
+          const int ln = sub_length(L, wv * kWaveRounds + r);
+          if (ln) or_field(sb, p0 + lane * PL * ln, lane_field(code[r], ln));
+          p0 += ln * Word<W>::PerSub;
+        }
+      } else {
+        // raw chunk: the original bytes (lossless codes invert exactly; lossy
+        // ones do not, so the input is read again)
 #pragma unroll
-        for (int k = 0; k < PS; k++) {
-          const W cv = to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
-          if (cv) or_code<W>(stage, p0 + k * ln, cv);
+        for (int r = 0; r < kWaveRounds; r++) {
+          uint2 q;
+          if constexpr (LOSSY) {
+            q = load_lane(a.in + base, woff + r * kSubBytes, osize);
+          } else {
+            W v[PL];
+#pragma unroll
+            for (int k = 0; k < PL; k++) v[k] = lossless_value<W>(code[r][k]);
+            q = values_lane(v);
+          }
+          *(uint2*)((uint8_t*)sb + woff + r * kSubBytes) = q;
         }
       }
-    } else {
-      // raw chunk: the original bytes
-#pragma unroll
-      for (int r = 0; r < kRounds; r++) ((uint4*)stage)[r * kThreads + t] = raw[r];
-    }
 
-    if (t < 64) {
-      // wave 0, after its share of the packing: the chunk's stream offset
-      const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
       if (t == 0) {
-        if (c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)csize);
-        s_excl = excl;
+        // this chunk completes its group: publish the group's aggregate
+        const long long g = c / kGroup;
+        const long long members = min((long long)kGroup, nchunks - g * kGroup);
+        if ((long long)(grp_old >> 32) + 1 == members)
+          flag_max(st.group + g, (g == 0 ? kInc : kAgg) | ((grp_old & 0xffffffffull) + (unsigned)csize));
       }
     }
-    __syncthreads();
 
-    const long long excl = uni((long long)s_excl);
-    store_unaligned(a.out + data_base + excl, stage, csize);
-    if (t == 0) {
-      if (c == 0) *(long long*)a.out = a.insize;
-      if (c == nchunks - 1) *a.outsize = data_base + excl + csize;
+    if (prev >= 0) {
+      if (t < 64) {
+        // wave 0: the stream offset of the previous chunk
+        const uint64_t excl = (prev == 0) ? 0ull : look_back<W, LOSSY>(a, st, prev, nchunks);
+        if (t == 0) {
+          if (prev != 0) publish_inclusive(st, prev, nchunks, excl + (uint64_t)prev_csize);
+          s_excl = excl;
+        }
+      }
+      __syncthreads();
+      const long long excl = uni((long long)s_excl);
+      store_unaligned(a.out + data_base + excl, stage[pb], prev_csize);
+      if (t == 0) {
+        if (prev == 0) *(long long*)a.out = a.insize;
+        if (prev == nchunks - 1) *a.outsize = data_base + excl + prev_csize;
+      }
     }
-    __syncthreads();   // stage is cleared for the next chunk only after every store read it
+    __syncthreads();   // stage[pb] is reused only after every store read it
+    if (!have) break;
+    prev = c;
+    prev_csize = csize;
+    pb = cb;
+    cb ^= 1;
   }
 }
 
@@ -423,13 +465,13 @@ __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 {
-  constexpr int PS = Word<W>::PerSlice;
+  constexpr int PL = Word<W>::PerLane;
   constexpr int TB = Word<W>::TB;
   __shared__ uint32_t stage[kStageBytes / 4];
 
   const int t = threadIdx.x;
-  const int lane32 = t & 31;
-  const int half = t >> 5;
+  const int lane = t & 63;
+  const int wv = t >> 6;
   const long long hdr = uni(*(const long long*)a.in);
   long long nchunks = 0;
   uint64_t why = 0;
@@ -485,16 +527,23 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
         if (corrupt) {
           if (t == 0) raise_status(a.ws, kStCorrupt);
         } else {
+          int p0 = 8 * (mis + kSub) + sub_bit_offset<W>(L, wv * kWaveRounds);
+          const int woff = wv * (kWaveRounds * kSubBytes) + lane * 8;
 #pragma unroll
-          for (int r = 0; r < kRounds; r++) {
-            const int off = r * kThreads * 16 + t * 16;
-            const int s = r * 8 + half;
-            const int ln = sub_length(L, s);
-            const int p0 = 8 * (mis + kSub) + sub_bit_offset<W>(L, s) + lane32 * PS * ln;
-            W v[PS];
-#pragma unroll
-            for (int k = 0; k < PS; k++) v[k] = from_code<W, LOSSY>(extract_code(stage, p0 + k * ln, ln, (W*)nullptr), a.thr_e, offs);
-            if (off < osize) store_slice(dst, off, osize, values_slice(v));
+          for (int r = 0; r < kWaveRounds; r++) {
+            const int ln = sub_length(L, wv * kWaveRounds + r);
+            const uint64_t f = read_field(stage, p0 + lane * PL * ln);
+            p0 += ln * Word<W>::PerSub;
+            W v[PL];
+            if constexpr (PL == 2) {
+              const uint32_t m = ln >= 32 ? 0xffffffffu : ((1u << ln) - 1u);
+              v[0] = from_code<W, LOSSY>((W)f & m, a.thr_e, offs);
+              v[1] = from_code<W, LOSSY>((W)(f >> ln) & m, a.thr_e, offs);
+            } else {
+              const uint64_t m = ln >= 64 ? ~0ull : ((1ull << ln) - 1ull);
+              v[0] = from_code<W, LOSSY>((W)(f & m), a.thr_e, offs);
+            }
+            store_lane(dst, woff + r * kSubBytes, osize, values_lane(v));
           }
         }
       }
@@ -510,7 +559,7 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 template <typename W, bool LOSSY>
 static hipError_t launch_encode(const EncodeArgs& a, int grid, int waves, hipStream_t stream)
 {
-  if (waves >= 8) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 8>), dim3(grid), dim3(kThreads), 0, stream, a);
+  if (waves >= 4) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 4>), dim3(grid), dim3(kThreads), 0, stream, a);
   else hipLaunchKernelGGL((encode_kernel<W, LOSSY, 1>), dim3(grid), dim3(kThreads), 0, stream, a);
   return hipGetLastError();
 }
@@ -538,7 +587,7 @@ hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int gr
 template <typename W, bool LOSSY>
 static const void* encode_fn(int waves)
 {
-  return waves >= 8 ? (const void*)encode_kernel<W, LOSSY, 8> : (const void*)encode_kernel<W, LOSSY, 1>;
+  return waves >= 4 ? (const void*)encode_kernel<W, LOSSY, 4> : (const void*)encode_kernel<W, LOSSY, 1>;
 }
 
 int resident_grid(int elem_bytes, bool lossy, bool decode, int waves)

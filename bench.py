@@ -36,7 +36,8 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     p.add_argument("--cpu-mb", type=int, default=64, help="CPU baseline sample size in MiB")
     p.add_argument("--seed", type=int, default=1)
-    p.add_argument("--encode-waves", type=int, default=0, help="encoder register budget (8 = 8 waves/SIMD)")
+    p.add_argument("--encode-variant", type=int, default=0,
+                   help="0 pipelined (default), 1 pipelined at the compiler's register choice, 2 one wave per chunk")
     p.add_argument("--encode-grid", type=int, default=0, help="encoder persistent grid (0 = automatic)")
     p.add_argument("--decode-grid", type=int, default=0, help="decoder grid (0 = one workgroup per chunk)")
     return p.parse_args()
@@ -90,10 +91,10 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    if args.encode_waves or args.encode_grid or args.decode_grid:
+    if args.encode_variant or args.encode_grid or args.decode_grid:
         from sleek_amd import _lib
 
-        _lib.check(_lib.load().sleek_gpu_set_tuning(args.encode_waves, args.encode_grid, args.decode_grid), "tuning")
+        _lib.check(_lib.load().sleek_gpu_set_tuning(args.encode_variant, args.encode_grid, args.decode_grid), "tuning")
     x = config_field(1, seed=args.seed, device=dev, scale=args.scale).reshape(-1).contiguous()
     nbytes = x.numel() * 4
     rng = finite_range(x)

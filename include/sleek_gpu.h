@@ -101,12 +101,13 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
                                const unsigned char* d_encoded, long long enclen, void* d_decoded,
                                long long capacity, long long* d_decsize, void* d_workspace,
                                long long workspace_bytes, int blocks, void* stream);
-/* Launch-shape knobs, found by the autotuner (sleek_amd/autotune.py); no
- * reference counterpart.  encode_waves: >= 4 = register budget for 4 waves
- * per SIMD, 1..3 = compiler's choice (0 = default, 4); encode_grid /
- * decode_grid: grid sizes (0 = automatic).  They change speed only, never
- * the bytes.  Applies to calls made after it returns. */
-int sleek_gpu_set_tuning(int encode_waves, int encode_grid, int decode_grid);
+/* Launch-shape knobs; no reference counterpart.  encode_variant: 0 =
+ * pipelined 256-thread encoder at a 4-waves-per-SIMD register budget (the
+ * default), 1 = the same at the compiler's register choice, 2 = one wave
+ * per chunk; encode_grid / decode_grid: grid sizes (0 = automatic).  They
+ * change speed only, never the bytes.  Applies to calls made after it
+ * returns; SLEEK_EINVAL for an unknown variant or a negative size. */
+int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid);
 
 /* Synchronises `stream`, then returns SLEEK_OK, SLEEK_ECORRUPT or
  * SLEEK_ECAPACITY from the workspace's status word. */

@@ -37,7 +37,7 @@ long long ws_bytes_for(long long insize)
 }
 
 // tuning knobs (sleek_gpu_set_tuning); none changes the bytes produced
-int g_encode_waves = 4;
+int g_encode_variant = sleek::kEncodePipelined;
 int g_encode_grid = 0;
 int g_decode_grid = 0;
 
@@ -76,7 +76,7 @@ int grid_for(int blocks, int elem_bytes, bool lossy, bool decode, long long nchu
 {
   long long g = blocks;
   if (g <= 0) g = decode ? g_decode_grid : g_encode_grid;
-  if (g <= 0) g = decode ? nchunks : sleek::resident_grid(elem_bytes, lossy, false, g_encode_waves);
+  if (g <= 0) g = decode ? nchunks : sleek::resident_grid(elem_bytes, lossy, false, g_encode_variant);
   if (g <= 0) g = 1024;
   if (nchunks >= 0 && g > nchunks) g = nchunks > 0 ? nchunks : 1;
   if (g > (1 << 20)) g = 1 << 20;
@@ -106,7 +106,7 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   a.thr_e = thr_e;
   a.offs = offs;
   const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
-  e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_waves, stream);
+  e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_variant, stream);
   if (e != hipSuccess) return hip_fail(e, "encode launch");
   return SLEEK_OK;
 }
@@ -210,7 +210,7 @@ int sleek_gpu_system_checks(void)
 
 int sleek_gpu_thread_blocks(void)
 {
-  const int g = sleek::resident_grid(4, false, false, g_encode_waves);
+  const int g = sleek::resident_grid(4, false, false, g_encode_variant);
   return g > 0 ? g : SLEEK_EHIP;
 }
 
@@ -305,11 +305,12 @@ int sleek_gpu_decompress_lossy_f64(int blocks, int eb_e, int thr_e, unsigned lon
   return decompress_sync(8, true, thr_e, offs_e, d_encoded, d_decoded, d_decsize, blocks);
 }
 
-int sleek_gpu_set_tuning(int encode_waves, int encode_grid, int decode_grid)
+int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid)
 {
-  if (encode_waves < 0 || encode_grid < 0 || decode_grid < 0) return SLEEK_EINVAL;
+  if (encode_variant < 0 || encode_variant >= sleek::kEncodeVariants || encode_grid < 0 || decode_grid < 0)
+    return SLEEK_EINVAL;
   std::lock_guard<std::mutex> lk(g_mu);
-  g_encode_waves = encode_waves > 0 ? encode_waves : 4;
+  g_encode_variant = encode_variant;
   g_encode_grid = encode_grid;
   g_decode_grid = decode_grid;
   return SLEEK_OK;

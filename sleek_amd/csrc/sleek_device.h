@@ -101,43 +101,37 @@ __device__ __forceinline__ W lossless_value(W v)
 // above 2^thr keep their bits (minus an offset), magnitudes in [2^eb, 2^thr)
 // are rounded to bins of width 2^(eb+1), smaller ones become 0; the sign is
 // folded into bit 0 with both zeros mapping to 0.  Unsigned arithmetic equals
-// the reference's wrapping signed arithmetic modulo 2^TB.
+// the reference's wrapping signed arithmetic modulo 2^TB.  Both arms are
+// computed and selected (shift amounts clamped to the word), so a wave never
+// diverges on the value's magnitude.
 template <typename W>
 __device__ __forceinline__ W lossy_code(W val, int eb_e, int thr_e, W offs)
 {
+  constexpr int TB = Word<W>::TB;
   constexpr int EM = Word<W>::EXPB + Word<W>::MANB;
   constexpr int M = Word<W>::MANB;
   const W mag = val & (((W)1 << EM) - 1);
   const int e = (int)(mag >> M);
-  W enc = 0;
-  if (e >= thr_e) {
-    enc = mag - offs;
-  } else if (e >= eb_e) {
-    const int sh = thr_e - e;
-    const W mant = (val & (((W)1 << M) - 1)) | ((W)1 << M);
-    enc = (mant + ((W)1 << (sh - 1))) >> sh;
-  }
+  const int sh = min(max(thr_e - e, 1), TB - 1);          // in [1, M+1] where it is used
+  const W mant = (val & (((W)1 << M) - 1)) | ((W)1 << M);
+  const W binned = (mant + ((W)1 << (sh - 1))) >> sh;
+  W enc = (e >= thr_e) ? (W)(mag - offs) : ((e >= eb_e) ? binned : (W)0);
   enc = (enc << 1) | ((~val) >> EM);
   return enc - (enc != 0 ? (W)1 : (W)0);
 }
 
-// Inverse of lossy_code (reference sleek_common_cpu.h:559-591).
+// Inverse of lossy_code (reference sleek_common_cpu.h:559-591), branch-free.
 template <typename W>
 __device__ __forceinline__ W lossy_value(W enc, int thr_e, W offs)
 {
   constexpr int TB = Word<W>::TB;
   constexpr int M = Word<W>::MANB;
   constexpr int EM = Word<W>::EXPB + Word<W>::MANB;
-  if (enc == 0) return 0;
   const W mag = (W)(enc + 1) >> 1;
-  W dec = 0;
-  if (mag >= ((W)1 << M)) {
-    dec = mag + offs;
-  } else {
-    const int sh = clz_w(mag) - (TB - 1 - M);
-    dec = ((mag << sh) & (((W)1 << M) - 1)) | ((W)(thr_e - sh) << M);
-  }
-  return dec | (enc << EM);
+  const int sh = min(max(clz_w(mag) - (TB - 1 - M), 0), TB - 1);   // in [1, M] where it is used
+  const W binned = ((mag << sh) & (((W)1 << M) - 1)) | ((W)(thr_e - sh) << M);
+  const W dec = (mag >= ((W)1 << M)) ? (W)(mag + offs) : binned;
+  return (enc == 0) ? (W)0 : (W)(dec | (enc << EM));
 }
 
 template <typename W, bool LOSSY>
@@ -242,6 +236,7 @@ __device__ __forceinline__ void reissue(uint4& q)
   asm volatile("" : "+v"(q.x), "+v"(q.y), "+v"(q.z), "+v"(q.w));
 }
 __device__ __forceinline__ void reissue(uint32_t& x) { asm volatile("" : "+v"(x)); }
+__device__ __forceinline__ void reissue_pair(uint2& q) { asm volatile("" : "+v"(q.x), "+v"(q.y)); }
 __device__ __forceinline__ void reissue(uint64_t& x)
 {
   uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
@@ -368,6 +363,27 @@ __device__ __forceinline__ uint4 lds_read16(const uint32_t* lds, int byte_off)
   const uint32_t x0 = lds[q], x1 = lds[q + 1], x2 = lds[q + 2], x3 = lds[q + 3], x4 = lds[q + 4];
   return make_uint4(__builtin_amdgcn_alignbit(x1, x0, sh), __builtin_amdgcn_alignbit(x2, x1, sh),
                     __builtin_amdgcn_alignbit(x3, x2, sh), __builtin_amdgcn_alignbit(x4, x3, sh));
+}
+
+// Copy len bytes from LDS to global memory at an arbitrary byte address, by
+// one wave (the single-wave variant of store_unaligned below).
+__device__ __forceinline__ void wave_store_unaligned(uint8_t* dst, const uint32_t* lds, int len)
+{
+  const int l = threadIdx.x & 63;
+  const int mis = (int)((uintptr_t)dst & 15);
+  const int head = min(len, (16 - mis) & 15);
+  const int nfull = (len - head) >> 4;
+  const int tail = len - head - (nfull << 4);
+  const uint8_t* lb = (const uint8_t*)lds;
+  if (l < head) dst[l] = lb[l];
+  if (l >= 16 && l < 16 + tail) {
+    const int i = head + (nfull << 4) + (l - 16);
+    dst[i] = lb[i];
+  }
+  for (int j = l; j < nfull; j += 64) {
+    const int p = head + (j << 4);
+    *(uint4*)(dst + p) = lds_read16(lds, p);
+  }
 }
 
 // Copy len bytes from LDS (byte 0 of lds) to global memory at an arbitrary

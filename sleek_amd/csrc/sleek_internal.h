@@ -27,12 +27,18 @@ struct DecodeArgs {
   unsigned long long offs;
 };
 
-// waves: occupancy target of the encoder (>= 4: register budget of 4 waves
-// per SIMD, the most its two LDS staging buffers allow; lower: the
-// compiler's choice).  Output bytes never depend
-// on it or on the grid size.
-hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int waves, hipStream_t stream);
+// Encoder variants.  kEncodePipelined: 256-thread workgroups, three-stage
+// pipeline, register budget of 4 waves per SIMD (the most its two LDS
+// staging buffers allow); kEncodeFree: the same kernel at the compiler's
+// register choice; kEncodeWave: one 64-thread wave per chunk.  Output bytes
+// never depend on the variant or on the grid size.
+constexpr int kEncodePipelined = 0;
+constexpr int kEncodeFree = 1;
+constexpr int kEncodeWave = 2;
+constexpr int kEncodeVariants = 3;
+
+hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);
 hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream);
-int resident_grid(int elem_bytes, bool lossy, bool decode, int waves);
+int resident_grid(int elem_bytes, bool lossy, bool decode, int variant);
 
 }  // namespace sleek

@@ -57,6 +57,21 @@ __device__ __forceinline__ uint2 load_lane(const uint8_t* src, int off, int osiz
   return q;
 }
 
+// Slice s (stride sub_stride 8-byte words) of a lane whose chunk has rem
+// valid bytes from the lane's first slice on; zero where invalid.  Written
+// with one lane pointer and constant offsets so that no per-slice offset is
+// kept in registers across the chunk loop.
+__device__ __forceinline__ uint2 load_lane_rem(const uint2* lp, int s, int sub_stride, int rem)
+{
+  const int b = s * sub_stride * 8;
+  uint2 q = make_uint2(0u, 0u);
+  if (rem > b) {
+    q = lp[s * sub_stride];
+    if (rem - b < 8) q.y = 0u;
+  }
+  return q;
+}
+
 __device__ __forceinline__ void store_lane(uint8_t* dst, int off, int osize, uint2 q)
 {
   if (off + 8 <= osize) *(uint2*)(dst + off) = q;
@@ -370,6 +385,131 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   }
 }
 
+// Encoder, one wave per chunk: a workgroup is a single wave that codes
+// its chunks (launch order, stride = grid) with no workgroup barrier at all.
+// Lane l holds the 8-byte slice l of each of the chunk's 32 subchunks, so
+// every per-subchunk quantity is wave-uniform; the chunk's raw slices stay
+// in registers (64 VGPRs) and its codes are computed twice (bit lengths,
+// then packing), which costs less than holding them as well.  Packing goes
+// through a private 16 KiB LDS buffer, which puts nine such waves on a CU;
+// the look-back and the load latencies of one wave are covered by the
+// others' arithmetic.
+template <typename W, bool LOSSY>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void encode_wave_kernel(EncodeArgs a)
+{
+  constexpr int PL = Word<W>::PerLane;
+  __shared__ uint32_t stage[kStageBytes / 4];
+
+  const int lane = threadIdx.x;
+  const long long nchunks = (a.insize + kChunk - 1) / kChunk;
+  const long long data_base = 8 + 2 * nchunks;
+  uint16_t* const sizes = (uint16_t*)(a.out + 8);
+  const ScanState st = scan_state(a.ws, nchunks);
+  const W offs = (W)a.offs;
+  const int loff = lane * 8;
+
+  if (nchunks == 0) {
+    if (blockIdx.x == 0 && lane == 0) {
+      *(long long*)a.out = 0;
+      *a.outsize = 8;
+    }
+    return;
+  }
+
+  long long c = blockIdx.x;
+  for (; c < nchunks; c += gridDim.x) {
+    for (int i = lane; i < kChunk / 16 + 1; i += 64) ((uint4*)stage)[i] = make_uint4(0u, 0u, 0u, 0u);
+    const long long base = c * kChunk;
+    const int osize = (int)min((long long)kChunk, a.insize - base);
+
+    // the chunk's raw slices stay in registers (64 VGPRs); codes are
+    // computed twice (bit lengths, then packing), which is cheaper than
+    // holding them next to the raw values
+    uint2 raw[kSub];
+    if (osize == kChunk) {
+      // full chunk: one pointer, constant offsets, no masking
+      const uint2* const p = (const uint2*)(a.in + base) + lane;
+#pragma unroll
+      for (int s = 0; s < kSub; s++) raw[s] = p[s * (kSubBytes / 8)];
+    } else {
+      const uint2* const p = (const uint2*)(a.in + base) + lane;
+      const int rem = osize - loff;
+#pragma unroll
+      for (int s = 0; s < kSub; s++) raw[s] = load_lane_rem(p, s, kSubBytes / 8, rem);
+    }
+
+    uint32_t L[8];   // bit lengths of the 32 subchunks, byte-packed (wave-uniform)
+#pragma unroll
+    for (int i = 0; i < 8; i++) L[i] = 0;
+#pragma unroll
+    for (int s = 0; s < kSub; s++) {
+      W v[PL];
+      lane_values(raw[s], v);
+      W orv = 0;
+#pragma unroll
+      for (int k = 0; k < PL; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
+      L[s / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (s % 4));
+    }
+    uint32_t nb = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) nb += byte_sum(L[i]);
+    const int total_bits = (int)nb * Word<W>::PerSub;
+    const int newsize = kSub + 2 + total_bits / 8;
+    const bool packed = newsize < osize;
+    const int csize = packed ? newsize : osize;
+
+    unsigned long long grp_old = 0;
+    if (lane == 0) {
+      flag_max(st.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
+      grp_old = __hip_atomic_fetch_add(st.count + c / kGroup, (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+      sizes[c] = (uint16_t)csize;
+    }
+    __syncthreads();   // stage cleared before it is packed into
+#pragma unroll
+    for (int s = 0; s < kSub; s++) reissue_pair(raw[s]);
+
+    if (packed) {
+      if (lane < 8) stage[lane] = L[lane];
+      if (lane == 8) stage[8 + total_bits / 32] = (uint32_t)osize;
+      int p0 = 8 * kSub;
+#pragma unroll
+      for (int s = 0; s < kSub; s++) {
+        const int ln = sub_length(L, s);
+        if (ln) {
+          W v[PL];
+          lane_values(raw[s], v);
+#pragma unroll
+          for (int k = 0; k < PL; k++) v[k] = to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
+          or_field(stage, p0 + lane * PL * ln, lane_field(v, ln));
+        }
+        p0 += ln * Word<W>::PerSub;
+      }
+    } else {
+      // raw chunk: the original bytes
+#pragma unroll
+      for (int s = 0; s < kSub; s++) ((uint2*)stage)[lane + s * (kSubBytes / 8)] = raw[s];
+    }
+
+    if (lane == 0) {
+      const long long g = c / kGroup;
+      const long long members = min((long long)kGroup, nchunks - g * kGroup);
+      if ((long long)(grp_old >> 32) + 1 == members)
+        flag_max(st.group + g, (g == 0 ? kInc : kAgg) | ((grp_old & 0xffffffffull) + (unsigned)csize));
+    }
+    const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
+    if (lane == 0 && c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)csize);
+    __syncthreads();   // packed payload complete
+
+    wave_store_unaligned(a.out + data_base + (long long)excl, stage, csize);
+    if (lane == 0) {
+      if (c == 0) *(long long*)a.out = a.insize;
+      if (c == nchunks - 1) *a.outsize = data_base + (long long)excl + csize;
+    }
+    __syncthreads();   // stage read by the stores before the next chunk clears it
+  }
+}
+
 // Validity of a stream header against the caller's limits (uniform).
 __device__ __forceinline__ bool header_ok(const DecodeArgs& a, long long hdr, long long* nchunks, uint64_t* why)
 {
@@ -557,10 +697,11 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 // ---- launchers ----
 
 template <typename W, bool LOSSY>
-static hipError_t launch_encode(const EncodeArgs& a, int grid, int waves, hipStream_t stream)
+static hipError_t launch_encode(const EncodeArgs& a, int grid, int variant, hipStream_t stream)
 {
-  if (waves >= 4) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 4>), dim3(grid), dim3(kThreads), 0, stream, a);
-  else hipLaunchKernelGGL((encode_kernel<W, LOSSY, 1>), dim3(grid), dim3(kThreads), 0, stream, a);
+  if (variant == kEncodeWave) hipLaunchKernelGGL((encode_wave_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
+  else if (variant == kEncodeFree) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 1>), dim3(grid), dim3(kThreads), 0, stream, a);
+  else hipLaunchKernelGGL((encode_kernel<W, LOSSY, 4>), dim3(grid), dim3(kThreads), 0, stream, a);
   return hipGetLastError();
 }
 
@@ -572,10 +713,10 @@ static hipError_t launch_decode(const DecodeArgs& a, int grid, hipStream_t strea
   return hipGetLastError();
 }
 
-hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int waves, hipStream_t stream)
+hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream)
 {
-  if (elem_bytes == 4) return lossy ? launch_encode<uint32_t, true>(a, grid, waves, stream) : launch_encode<uint32_t, false>(a, grid, waves, stream);
-  return lossy ? launch_encode<uint64_t, true>(a, grid, waves, stream) : launch_encode<uint64_t, false>(a, grid, waves, stream);
+  if (elem_bytes == 4) return lossy ? launch_encode<uint32_t, true>(a, grid, variant, stream) : launch_encode<uint32_t, false>(a, grid, variant, stream);
+  return lossy ? launch_encode<uint64_t, true>(a, grid, variant, stream) : launch_encode<uint64_t, false>(a, grid, variant, stream);
 }
 
 hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream)
@@ -585,12 +726,13 @@ hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int gr
 }
 
 template <typename W, bool LOSSY>
-static const void* encode_fn(int waves)
+static const void* encode_fn(int variant)
 {
-  return waves >= 4 ? (const void*)encode_kernel<W, LOSSY, 4> : (const void*)encode_kernel<W, LOSSY, 1>;
+  if (variant == kEncodeWave) return (const void*)encode_wave_kernel<W, LOSSY>;
+  return variant == kEncodeFree ? (const void*)encode_kernel<W, LOSSY, 1> : (const void*)encode_kernel<W, LOSSY, 4>;
 }
 
-int resident_grid(int elem_bytes, bool lossy, bool decode, int waves)
+int resident_grid(int elem_bytes, bool lossy, bool decode, int variant)
 {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
@@ -600,10 +742,11 @@ int resident_grid(int elem_bytes, bool lossy, bool decode, int waves)
     fn = elem_bytes == 4 ? (lossy ? (const void*)decode_kernel<uint32_t, true> : (const void*)decode_kernel<uint32_t, false>)
                          : (lossy ? (const void*)decode_kernel<uint64_t, true> : (const void*)decode_kernel<uint64_t, false>);
   } else {
-    fn = elem_bytes == 4 ? (lossy ? encode_fn<uint32_t, true>(waves) : encode_fn<uint32_t, false>(waves))
-                         : (lossy ? encode_fn<uint64_t, true>(waves) : encode_fn<uint64_t, false>(waves));
+    fn = elem_bytes == 4 ? (lossy ? encode_fn<uint32_t, true>(variant) : encode_fn<uint32_t, false>(variant))
+                         : (lossy ? encode_fn<uint64_t, true>(variant) : encode_fn<uint64_t, false>(variant));
   }
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, kThreads, 0) != hipSuccess) return 0;
+  const int threads = (!decode && variant == kEncodeWave) ? 64 : kThreads;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);
 }
 

@@ -104,6 +104,31 @@ def test_blocks_invariance(gpu, dt, blocks):
             assert rc == 0 and d.tobytes() == od.tobytes()
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("variant", [0, 1, 2])
+def test_encoder_variant_invariance(gpu, dt, variant):
+    """Every encoder kernel (sleek_gpu_set_tuning's encode_variant) writes the oracle's bytes,
+    at the automatic grid and at a 5-workgroup grid that makes each one walk many chunks."""
+    say(f"encoder variant={variant} {np.dtype(dt).name}")
+    from sleek_amd import _lib
+    from sleek_amd.datagen import exact_field, inject_specials
+
+    lib = _lib.load()
+    a = exact_field(400_003 if dt == np.float32 else 200_003, 5, dt)
+    inject_specials(a, 1e-3, 11)
+    try:
+        for grid in (0, 5):
+            _lib.check(lib.sleek_gpu_set_tuning(variant, grid, 0), "tuning")
+            for eb in (None, 1e-2, 1e-5):
+                ref = po.compress(a, eb)
+                s, d = gpu_roundtrip(a, eb)
+                assert s.tobytes() == ref.tobytes(), (grid, eb)
+                od, rc = po.decompress(ref, dt, eb)
+                assert rc == 0 and d.tobytes() == od.tobytes()
+    finally:
+        lib.sleek_gpu_set_tuning(0, 0, 0)
+
+
 @pytest.mark.parametrize("config", [1, 2])
 def test_config_fields_scaled(gpu, config):
     """The benchmark recipes (sinusoids, ramp, lognormal noise, specials) at 1/4 scale."""

@@ -113,6 +113,62 @@ int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid);
  * SLEEK_ECAPACITY from the workspace's status word. */
 int sleek_gpu_workspace_status(const void* d_workspace, void* stream);
 
+/* ---- batched calls: many arrays in one launch per direction ----
+ * No reference counterpart (the reference codes one array per call).  Every
+ * array's stream / decoded bytes are exactly those of the single-array
+ * calls.  Arrays may mix float32 and float64, lossless and lossy.
+ *
+ * The caller fills the first block of each item; sleek_gpu_batch_plan
+ * fills the second and returns the tile count and workspace size.  The
+ * caller then copies the planned items to device memory (d_items) and
+ * launches with a workspace of at least that size, which must not be shared
+ * by launches that may overlap.  A tile is one 16 KiB chunk of one array;
+ * workgroups take tiles in launch order through a device tile map. */
+typedef struct sleek_gpu_batch_item {
+  const void* src;          /* compress: input array (16-byte aligned); decompress: stream (8-byte aligned) */
+  void* dst;                /* compress: stream buffer of sleek_gpu_upper_bound_size(src_bytes) bytes
+                               (8-byte aligned); decompress: output array (16-byte aligned) */
+  long long src_bytes;      /* compress: input bytes (a multiple of elem_bytes);
+                               decompress: stream bytes available, or -1 if unknown */
+  long long dst_bytes;      /* decompress: capacity in bytes (required); compress: ignored */
+  long long* d_size;        /* device int64 result: stream length / decoded length */
+  unsigned long long offs;  /* lossy parameters, from sleek_gpu_error_bound_f32/_f64 */
+  int eb_e, thr_e;
+  int elem_bytes;           /* 4 (float32) or 8 (float64) */
+  int lossy;                /* 0 or 1 */
+  /* filled by sleek_gpu_batch_plan */
+  long long tile0;          /* first tile of this array */
+  long long tiles;          /* its tiles (>= 1) */
+  long long ws_word;        /* first 8-byte word of its workspace block */
+  long long ws_words;       /* words in that block */
+} sleek_gpu_batch_item;     /* 96 bytes */
+
+/* Plans n items for compression (decompress = 0) or decompression (1).
+ * SLEEK_EINVAL names nothing: a bad item (size, type, missing capacity),
+ * SLEEK_EALIGN a misaligned pointer. */
+int sleek_gpu_batch_plan(sleek_gpu_batch_item* items, int n, int decompress, long long* tiles,
+                         long long* workspace_bytes);
+/* One launch sequence per direction on `stream`: a preparation kernel
+ * (clears each array's workspace block with stores, writes the tile map)
+ * and the coder; the decoder adds a per-array offset scan.  grid <= 0
+ * picks a persistent grid.  No allocation, no memset: capturable. */
+int sleek_gpu_batch_compress_async(const sleek_gpu_batch_item* d_items, int n, long long tiles, void* d_workspace,
+                                   long long workspace_bytes, int grid, void* stream);
+int sleek_gpu_batch_decompress_async(const sleek_gpu_batch_item* d_items, int n, long long tiles,
+                                     void* d_workspace, long long workspace_bytes, int grid, void* stream);
+/* Per-array status after a batched launch: synchronises `stream`, then
+ * writes SLEEK_OK / SLEEK_ECORRUPT / SLEEK_ECAPACITY into status[0..n). */
+int sleek_gpu_batch_status(const sleek_gpu_batch_item* d_items, int n, void* d_workspace, long long workspace_bytes,
+                           int* status, void* stream);
+/* The batched launch sequence captured once into an executable HIP graph
+ * (*graph), replayed by sleek_gpu_graph_launch on any stream; the items,
+ * buffers and workspace it names must stay allocated until
+ * sleek_gpu_graph_destroy. */
+int sleek_gpu_batch_graph_create(int decompress, const sleek_gpu_batch_item* d_items, int n, long long tiles,
+                                 void* d_workspace, long long workspace_bytes, int grid, void** graph);
+int sleek_gpu_graph_launch(void* graph, void* stream);
+int sleek_gpu_graph_destroy(void* graph);
+
 #ifdef __cplusplus
 }
 #endif

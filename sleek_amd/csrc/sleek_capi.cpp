@@ -7,6 +7,7 @@
 // stream-ordered and take a caller-owned workspace.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <stddef.h>
 #include <string.h>
 
 #include <mutex>
@@ -29,12 +30,14 @@ long long chunks_of(long long bytes) { return (bytes + SLEEK_CHUNK_BYTES - 1) / 
 
 // workspace: 8 header words (word 1: status), then per chunk one word, and
 // per group of 64 chunks two words (look-back state when encoding; chunk
-// offsets and their total when decoding)
-long long ws_bytes_for(long long insize)
+// offsets and their total when decoding); 8 spare words, an even total
+long long ws_words_for(long long insize)
 {
   const long long ch = chunks_of(insize);
-  return (16 + ch + 2 * ((ch + 63) / 64)) * 8;
+  return sleek::even_words(16 + ch + 2 * ((ch + 63) / 64));
 }
+
+long long ws_bytes_for(long long insize) { return ws_words_for(insize) * 8; }
 
 // tuning knobs (sleek_gpu_set_tuning); none changes the bytes produced
 int g_encode_variant = sleek::kEncodePipelined;
@@ -94,8 +97,9 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   if (!aligned(d_input, 16) || !aligned(d_encoded, 8) || !aligned(d_encsize, 8) || !aligned(ws, 8)) return SLEEK_EALIGN;
   const long long need = ws_bytes_for(insize);
   if (ws_bytes < need) return SLEEK_EWORKSPACE;
-  hipError_t e = hipMemsetAsync(ws, 0, (size_t)need, stream);
-  if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  // cleared with stores, not a memset, so that callers may capture the call
+  hipError_t e = sleek::zero_launch((uint64_t*)ws, need / 8, stream);
+  if (e != hipSuccess) return hip_fail(e, "workspace clear");
   sleek::EncodeArgs a;
   a.in = (const uint8_t*)d_input;
   a.insize = insize;
@@ -126,8 +130,8 @@ int decompress_impl(int elem_bytes, bool lossy, int thr_e, unsigned long long of
   const long long cap = capacity >= 0 ? capacity : (ws_chunks - 8) * SLEEK_CHUNK_BYTES;
   if (cap < 0) return SLEEK_EWORKSPACE;
   // only the status word needs clearing: the offsets are written
-  hipError_t e = hipMemsetAsync(ws, 0, 64, stream);
-  if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+  hipError_t e = sleek::zero_launch((uint64_t*)ws, 8, stream);
+  if (e != hipSuccess) return hip_fail(e, "workspace clear");
   sleek::DecodeArgs a;
   a.in = d_encoded;
   a.inlen = enclen;
@@ -343,6 +347,140 @@ int sleek_gpu_workspace_status(const void* d_workspace, void* stream)
 {
   if (!d_workspace) return SLEEK_EINVAL;
   return read_status(d_workspace, (hipStream_t)stream);
+}
+
+// ---- batched calls ----
+
+static_assert(sizeof(sleek_gpu_batch_item) == sizeof(sleek::BatchItem), "batch item layout");
+static_assert(offsetof(sleek_gpu_batch_item, offs) == offsetof(sleek::BatchItem, offs), "batch item layout");
+static_assert(offsetof(sleek_gpu_batch_item, lossy) == offsetof(sleek::BatchItem, lossy), "batch item layout");
+static_assert(offsetof(sleek_gpu_batch_item, ws_words) == offsetof(sleek::BatchItem, ws_words), "batch item layout");
+
+int sleek_gpu_batch_plan(sleek_gpu_batch_item* items, int n, int decompress, long long* tiles, long long* workspace_bytes)
+{
+  if (n < 0 || (n > 0 && !items) || !tiles || !workspace_bytes) return SLEEK_EINVAL;
+  long long tile = 0;
+  for (int i = 0; i < n; i++) {
+    sleek_gpu_batch_item& it = items[i];
+    if ((it.elem_bytes != 4 && it.elem_bytes != 8) || (it.lossy != 0 && it.lossy != 1) || !it.d_size)
+      return SLEEK_EINVAL;
+    // an empty array may come with a null data pointer
+    if (decompress ? (!it.src || (!it.dst && it.dst_bytes > 0)) : (!it.dst || (!it.src && it.src_bytes > 0)))
+      return SLEEK_EINVAL;
+    if (!aligned(it.d_size, 8)) return SLEEK_EALIGN;
+    long long bytes;
+    if (decompress) {
+      if (it.dst_bytes < 0 || (it.src_bytes >= 0 && it.src_bytes < 8)) return SLEEK_EINVAL;
+      if (!aligned(it.src, 8) || !aligned(it.dst, 16)) return SLEEK_EALIGN;
+      bytes = it.dst_bytes;
+    } else {
+      if (it.src_bytes < 0 || it.src_bytes % it.elem_bytes != 0) return SLEEK_EINVAL;
+      if (!aligned(it.src, 16) || !aligned(it.dst, 8)) return SLEEK_EALIGN;
+      bytes = it.src_bytes;
+    }
+    const long long ch = chunks_of(bytes);
+    it.tile0 = tile;
+    it.tiles = ch > 0 ? ch : 1;
+    it.ws_words = ws_words_for(bytes);
+    tile += it.tiles;
+    if (tile > 0xffffffffll) return SLEEK_EINVAL;
+  }
+  // blocks follow the header, the status words and the tile map
+  long long word = sleek::batch_map_word(n) + sleek::even_words((tile + 1) / 2);
+  for (int i = 0; i < n; i++) {
+    items[i].ws_word = word;
+    word += items[i].ws_words;
+  }
+  *tiles = tile;
+  *workspace_bytes = word * 8;
+  return SLEEK_OK;
+}
+
+static int batch_async(bool decompress, const sleek_gpu_batch_item* d_items, int n, long long tiles, void* ws,
+                       long long ws_bytes, int grid, void* stream)
+{
+  if (n < 0 || tiles < 0 || (n > 0 && (!d_items || !ws || tiles < n))) return SLEEK_EINVAL;
+  if (n == 0) return SLEEK_OK;
+  if (!aligned(ws, 16) || !aligned(d_items, 8)) return SLEEK_EALIGN;
+  if (ws_bytes < (sleek::batch_map_word(n) + (tiles + 1) / 2) * 8) return SLEEK_EWORKSPACE;
+  const hipError_t e = sleek::batch_launch(decompress, (const sleek::BatchItem*)d_items, n, tiles, (uint64_t*)ws, grid,
+                                           (hipStream_t)stream);
+  return e == hipSuccess ? SLEEK_OK : hip_fail(e, "batch launch");
+}
+
+int sleek_gpu_batch_compress_async(const sleek_gpu_batch_item* d_items, int n, long long tiles, void* d_workspace,
+                                   long long workspace_bytes, int grid, void* stream)
+{
+  return batch_async(false, d_items, n, tiles, d_workspace, workspace_bytes, grid, stream);
+}
+
+int sleek_gpu_batch_decompress_async(const sleek_gpu_batch_item* d_items, int n, long long tiles, void* d_workspace,
+                                     long long workspace_bytes, int grid, void* stream)
+{
+  return batch_async(true, d_items, n, tiles, d_workspace, workspace_bytes, grid, stream);
+}
+
+int sleek_gpu_batch_status(const sleek_gpu_batch_item* d_items, int n, void* d_workspace, long long workspace_bytes,
+                           int* status, void* stream)
+{
+  if (n < 0 || (n > 0 && (!d_items || !d_workspace || !status))) return SLEEK_EINVAL;
+  if (n == 0) return SLEEK_OK;
+  if (workspace_bytes < (sleek::kBatchHead + n) * 8) return SLEEK_EWORKSPACE;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = sleek::batch_status_launch((const sleek::BatchItem*)d_items, n, (uint64_t*)d_workspace, s);
+  if (e != hipSuccess) return hip_fail(e, "batch status");
+  std::vector<unsigned long long> words((size_t)n);
+  e = hipMemcpyAsync(words.data(), (const uint64_t*)d_workspace + sleek::kBatchHead, (size_t)n * 8,
+                     hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync status");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
+  for (int i = 0; i < n; i++)
+    status[i] = (words[i] & 2) ? SLEEK_ECAPACITY : ((words[i] & 1) ? SLEEK_ECORRUPT : SLEEK_OK);
+  return SLEEK_OK;
+}
+
+int sleek_gpu_batch_graph_create(int decompress, const sleek_gpu_batch_item* d_items, int n, long long tiles,
+                                 void* d_workspace, long long workspace_bytes, int grid, void** graph)
+{
+  if (!graph) return SLEEK_EINVAL;
+  *graph = nullptr;
+  hipStream_t s = nullptr;
+  hipError_t e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+  if (e != hipSuccess) return hip_fail(e, "hipStreamCreate");
+  hipGraph_t g = nullptr;
+  hipGraphExec_t x = nullptr;
+  int rc = SLEEK_OK;
+  e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    rc = hip_fail(e, "hipStreamBeginCapture");
+  } else {
+    rc = batch_async(decompress != 0, d_items, n, tiles, d_workspace, workspace_bytes, grid, s);
+    e = hipStreamEndCapture(s, &g);
+    if (rc == SLEEK_OK && e != hipSuccess) rc = hip_fail(e, "hipStreamEndCapture");
+    if (rc == SLEEK_OK) {
+      e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+      if (e != hipSuccess) rc = hip_fail(e, "hipGraphInstantiate");
+    }
+  }
+  if (g) (void)hipGraphDestroy(g);
+  (void)hipStreamDestroy(s);
+  if (rc == SLEEK_OK) *graph = (void*)x;
+  return rc;
+}
+
+int sleek_gpu_graph_launch(void* graph, void* stream)
+{
+  if (!graph) return SLEEK_EINVAL;
+  const hipError_t e = hipGraphLaunch((hipGraphExec_t)graph, (hipStream_t)stream);
+  return e == hipSuccess ? SLEEK_OK : hip_fail(e, "hipGraphLaunch");
+}
+
+int sleek_gpu_graph_destroy(void* graph)
+{
+  if (!graph) return SLEEK_OK;
+  const hipError_t e = hipGraphExecDestroy((hipGraphExec_t)graph);
+  return e == hipSuccess ? SLEEK_OK : hip_fail(e, "hipGraphExecDestroy");
 }
 
 }  // extern "C"

@@ -38,7 +38,40 @@ constexpr int kEncodeWave = 2;
 constexpr int kEncodeVariants = 3;
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);
+// Clears `words` 8-byte words with vector stores (no memset node when the
+// caller captures a graph).  p is 8-byte aligned.
+hipError_t zero_launch(uint64_t* p, long long words, hipStream_t stream);
 hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream);
 int resident_grid(int elem_bytes, bool lossy, bool decode, int variant);
+
+// One array of a batched call; the layout of sleek_gpu_batch_item.
+struct BatchItem {
+  const uint8_t* src;
+  uint8_t* dst;
+  long long src_bytes;
+  long long dst_bytes;
+  long long* d_size;
+  unsigned long long offs;
+  int eb_e, thr_e;
+  int elem_bytes;
+  int lossy;
+  long long tile0;
+  long long tiles;
+  long long ws_word;
+  long long ws_words;
+};
+static_assert(sizeof(BatchItem) == 96, "BatchItem layout");
+
+// Batch workspace: kBatchHead header words, one status word per array,
+// the tile map (uint32 per tile), then each array's block (same layout as a
+// single-array workspace).  All regions start at even words (16 bytes).
+constexpr long long kBatchHead = 16;
+inline long long even_words(long long w) { return (w + 1) & ~1ll; }
+inline long long batch_map_word(int n) { return kBatchHead + even_words(n); }
+
+hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
+                        hipStream_t stream);
+hipError_t batch_status_launch(const BatchItem* items, int n, uint64_t* ws, hipStream_t stream);
+int batch_resident_grid(bool decompress);
 
 }  // namespace sleek

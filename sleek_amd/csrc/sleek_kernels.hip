@@ -14,199 +14,9 @@
 //   coalesced stores.
 // Reference counterparts: d_encode / d_decode, sleek_common_gpu.h:474-768
 // (lossless) and :989-1279 (lossy).
-#include "sleek_device.h"
-#include "sleek_internal.h"
+#include "sleek_chunk.h"
 
 namespace sleek {
-
-// Load the 16-byte slice at byte off of a chunk of osize bytes, zero beyond
-// osize.  The slice is 16-byte aligned, so when any of its bytes is valid the
-// whole slice lies in a mapped page and can be loaded at once.
-__device__ __forceinline__ uint4 load_slice(const uint8_t* src, int off, int osize)
-{
-  if (off >= osize) return make_uint4(0u, 0u, 0u, 0u);
-  uint4 q = *(const uint4*)(src + off);
-  const int valid = osize - off;   // bytes; a multiple of 4
-  if (valid < 16) {
-    q.w = 0u;
-    if (valid < 12) q.z = 0u;
-    if (valid < 8) q.y = 0u;
-  }
-  return q;
-}
-
-__device__ __forceinline__ void store_slice(uint8_t* dst, int off, int osize, uint4 q)
-{
-  if (off + 16 <= osize) {
-    *(uint4*)(dst + off) = q;
-    return;
-  }
-  const uint32_t d[4] = {q.x, q.y, q.z, q.w};
-#pragma unroll
-  for (int i = 0; i < 4; i++)
-    if (off + 4 * i + 4 <= osize) ((uint32_t*)(dst + off))[i] = d[i];
-}
-
-// The 8-byte slice at byte off of a chunk of osize bytes (osize is a
-// multiple of 4), zero beyond osize; aligned, so it never crosses a page.
-__device__ __forceinline__ uint2 load_lane(const uint8_t* src, int off, int osize)
-{
-  if (off >= osize) return make_uint2(0u, 0u);
-  uint2 q = *(const uint2*)(src + off);
-  if (osize - off < 8) q.y = 0u;
-  return q;
-}
-
-// Slice s (stride sub_stride 8-byte words) of a lane whose chunk has rem
-// valid bytes from the lane's first slice on; zero where invalid.  Written
-// with one lane pointer and constant offsets so that no per-slice offset is
-// kept in registers across the chunk loop.
-__device__ __forceinline__ uint2 load_lane_rem(const uint2* lp, int s, int sub_stride, int rem)
-{
-  const int b = s * sub_stride * 8;
-  uint2 q = make_uint2(0u, 0u);
-  if (rem > b) {
-    q = lp[s * sub_stride];
-    if (rem - b < 8) q.y = 0u;
-  }
-  return q;
-}
-
-__device__ __forceinline__ void store_lane(uint8_t* dst, int off, int osize, uint2 q)
-{
-  if (off + 8 <= osize) *(uint2*)(dst + off) = q;
-  else if (off < osize) *(uint32_t*)(dst + off) = q.x;
-}
-
-// Compressed size of chunk j, computed by one whole wave from the input (the
-// same bit lengths the chunk's own workgroup finds).  Used by a look-back
-// that has waited too long on an unpublished predecessor.
-template <typename W, bool LOSSY>
-__device__ uint32_t wave_chunk_size(const EncodeArgs& a, long long j)
-{
-  const int lane = threadIdx.x & 63;
-  const long long base = j * kChunk;
-  const int osize = (int)min((long long)kChunk, a.insize - base);
-  const uint8_t* const src = a.in + base;
-  uint32_t nb = 0;
-  for (int s = 0; s < kSub; s++) {
-    W v[Word<W>::PerLane];
-    lane_values(load_lane(src, s * kSubBytes + lane * 8, osize), v);
-    W orv = 0;
-#pragma unroll
-    for (int k = 0; k < Word<W>::PerLane; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, (W)a.offs);
-    nb += (uint32_t)wave_bit_length(orv);
-  }
-  const int newsize = kSub + 2 + (int)(nb * Word<W>::PerSub / 8);
-  return (uint32_t)(newsize < osize ? newsize : osize);
-}
-
-// Compressed size of group gj (sum over its chunks), by one whole wave:
-// published aggregate words are used, every other chunk is sized from the
-// input.
-template <typename W, bool LOSSY>
-__device__ uint64_t wave_group_size(const EncodeArgs& a, const ScanState& s, long long gj, long long nchunks)
-{
-  const int lane = threadIdx.x & 63;
-  const long long first = gj * kGroup;
-  const int members = (int)min((long long)kGroup, nchunks - first);
-  uint64_t f = (lane < members) ? flag_load(s.chunk + first + lane) : kAgg;
-  uint64_t mine = ((f >> 62) == 1) ? (f & kValMask) : 0;
-  uint64_t todo = __ballot(lane < members && (f >> 62) != 1);
-  while (todo) {
-    const int l = __ffsll((unsigned long long)todo) - 1;
-    const uint32_t sz = wave_chunk_size<W, LOSSY>(a, first + l);
-    if (lane == l) mine = sz;
-    todo &= todo - 1;
-  }
-  return wave_sum(mine);
-}
-
-// Help threshold of a look-back: after this long (100 MHz real-time ticks)
-// without progress, the wave sizes the blocking chunk or group itself.
-constexpr uint64_t kHelpTicks = 10000;   // 100 us
-
-// Two-level decoupled look-back, run by one whole wave; returns the sum of
-// the compressed sizes of chunks 0..c-1 in every lane.  First the preceding
-// chunks of c's own group are watched (one lane each); once they are all
-// published, a window of earlier groups is watched, starting 4 groups wide
-// and doubling to 64 as it slides back, so the inclusive frontier can
-// advance up to 4096 chunks per round trip while a poll reads few words.
-// Polls back off exponentially: the words live at device scope, and many
-// waiting waves polling at full rate would compete with the data traffic.
-// Workgroups take chunks in launch order, so the words waited on are
-// normally published within a few round trips; a word still missing after
-// kHelpTicks is computed from the input and published by the waiter, so
-// the look-back finishes even when a predecessor's workgroup is not running.
-template <typename W, bool LOSSY>
-__device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long c, long long nchunks)
-{
-  const int lane = threadIdx.x & 63;
-  const int q = (int)(c % kGroup);                 // chunks of the group before c
-  long long gend = c / kGroup;                      // group window: groups gend-gwin .. gend-1
-  int gwin = 4;
-  bool have_partial = (q == 0);
-  uint64_t partial = 0, gacc = 0;
-  uint64_t since = __builtin_amdgcn_s_memrealtime();
-  int naps = 1;
-  while (true) {
-    bool progressed = false;
-    if (!have_partial) {
-      const uint64_t f = (lane < q) ? flag_load(s.chunk + (c - 1 - lane)) : kAgg;   // beyond the group: 0
-      const uint64_t inc = __ballot((f >> 62) == 2);
-      const uint64_t rdy = __ballot((f >> 62) != 0);
-      if (inc) {
-        const int p = __ffsll((unsigned long long)inc) - 1;
-        const uint64_t need = (2ull << p) - 1;      // p < 63 here
-        if ((rdy & need) == need) return wave_sum(lane <= p ? (f & kValMask) : 0);
-      } else if (rdy == ~0ull) {
-        partial = wave_sum(f & kValMask);
-        have_partial = true;
-        progressed = true;
-      }
-      if (!progressed && __builtin_amdgcn_s_memrealtime() - since > kHelpTicks) {
-        const uint64_t miss = __ballot(lane < q && (f >> 62) == 0);
-        if (miss) {
-          const long long jc = c - __ffsll((unsigned long long)miss);
-          const uint32_t sz = wave_chunk_size<W, LOSSY>(a, jc);
-          if (lane == 0) flag_max(s.chunk + jc, kAgg | sz);
-        }
-        progressed = true;
-      }
-    } else {
-      const long long gj = gend - 1 - lane;
-      const uint64_t fg = (lane >= gwin) ? kAgg : ((gj >= 0) ? flag_load(s.group + gj) : kInc);  // before group 0: inclusive 0
-      const uint64_t inc = __ballot((fg >> 62) == 2);
-      const uint64_t rdy = __ballot((fg >> 62) != 0);
-      if (inc) {
-        const int p = __ffsll((unsigned long long)inc) - 1;
-        const uint64_t need = (2ull << p) - 1;      // p < gwin <= 64; lanes >= gwin carry 0
-        if ((rdy & need) == need) return partial + gacc + wave_sum(lane <= p ? (fg & kValMask) : 0);
-      } else if (rdy == ~0ull) {
-        gacc += wave_sum(fg & kValMask);
-        gend -= gwin;
-        if (gwin < 64) gwin *= 2;
-        progressed = true;
-      }
-      if (!progressed && __builtin_amdgcn_s_memrealtime() - since > kHelpTicks) {
-        const uint64_t miss = __ballot((fg >> 62) == 0);
-        if (miss) {
-          const long long gm = gend - __ffsll((unsigned long long)miss);
-          const uint64_t sz = wave_group_size<W, LOSSY>(a, s, gm, nchunks);
-          if (lane == 0) flag_max(s.group + gm, (gm == 0 ? kInc : kAgg) | sz);
-        }
-        progressed = true;
-      }
-    }
-    if (progressed) {
-      since = __builtin_amdgcn_s_memrealtime();
-      naps = 1;
-      continue;
-    }
-    for (int i = 0; i < naps; i++) __builtin_amdgcn_s_sleep(2);
-    if (naps < 32) naps *= 2;
-  }
-}
 
 // Encoder: each workgroup walks its chunks (launch order, stride = grid) as
 // a three-stage pipeline, one chunk per stage per iteration:
@@ -385,313 +195,42 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   }
 }
 
-// Encoder, one wave per chunk: a workgroup is a single wave that codes
-// its chunks (launch order, stride = grid) with no workgroup barrier at all.
-// Lane l holds the 8-byte slice l of each of the chunk's 32 subchunks, so
-// every per-subchunk quantity is wave-uniform; the chunk's raw slices stay
-// in registers (64 VGPRs) and its codes are computed twice (bit lengths,
-// then packing), which costs less than holding them as well.  Packing goes
-// through a private 16 KiB LDS buffer, which puts nine such waves on a CU;
-// the look-back and the load latencies of one wave are covered by the
-// others' arithmetic.
+// Encoder, one wave per chunk: a workgroup is a single wave that codes its
+// chunks (launch order, stride = grid) with encode_chunk_wave.  The private
+// 16 KiB LDS buffer puts nine such waves on a CU; the look-back and the
+// load latencies of one wave are covered by the others' arithmetic.
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void encode_wave_kernel(EncodeArgs a)
 {
-  constexpr int PL = Word<W>::PerLane;
   __shared__ uint32_t stage[kStageBytes / 4];
-
-  const int lane = threadIdx.x;
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
-  const long long data_base = 8 + 2 * nchunks;
-  uint16_t* const sizes = (uint16_t*)(a.out + 8);
-  const ScanState st = scan_state(a.ws, nchunks);
-  const W offs = (W)a.offs;
-  const int loff = lane * 8;
-
   if (nchunks == 0) {
-    if (blockIdx.x == 0 && lane == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
       *(long long*)a.out = 0;
       *a.outsize = 8;
     }
     return;
   }
-
-  long long c = blockIdx.x;
-  for (; c < nchunks; c += gridDim.x) {
-    for (int i = lane; i < kChunk / 16 + 1; i += 64) ((uint4*)stage)[i] = make_uint4(0u, 0u, 0u, 0u);
-    const long long base = c * kChunk;
-    const int osize = (int)min((long long)kChunk, a.insize - base);
-
-    // the chunk's raw slices stay in registers (64 VGPRs); codes are
-    // computed twice (bit lengths, then packing), which is cheaper than
-    // holding them next to the raw values
-    uint2 raw[kSub];
-    if (osize == kChunk) {
-      // full chunk: one pointer, constant offsets, no masking
-      const uint2* const p = (const uint2*)(a.in + base) + lane;
-#pragma unroll
-      for (int s = 0; s < kSub; s++) raw[s] = p[s * (kSubBytes / 8)];
-    } else {
-      const uint2* const p = (const uint2*)(a.in + base) + lane;
-      const int rem = osize - loff;
-#pragma unroll
-      for (int s = 0; s < kSub; s++) raw[s] = load_lane_rem(p, s, kSubBytes / 8, rem);
-    }
-
-    uint32_t L[8];   // bit lengths of the 32 subchunks, byte-packed (wave-uniform)
-#pragma unroll
-    for (int i = 0; i < 8; i++) L[i] = 0;
-#pragma unroll
-    for (int s = 0; s < kSub; s++) {
-      W v[PL];
-      lane_values(raw[s], v);
-      W orv = 0;
-#pragma unroll
-      for (int k = 0; k < PL; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
-      L[s / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (s % 4));
-    }
-    uint32_t nb = 0;
-#pragma unroll
-    for (int i = 0; i < 8; i++) nb += byte_sum(L[i]);
-    const int total_bits = (int)nb * Word<W>::PerSub;
-    const int newsize = kSub + 2 + total_bits / 8;
-    const bool packed = newsize < osize;
-    const int csize = packed ? newsize : osize;
-
-    unsigned long long grp_old = 0;
-    if (lane == 0) {
-      flag_max(st.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
-      grp_old = __hip_atomic_fetch_add(st.count + c / kGroup, (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-      sizes[c] = (uint16_t)csize;
-    }
-    __syncthreads();   // stage cleared before it is packed into
-#pragma unroll
-    for (int s = 0; s < kSub; s++) reissue_pair(raw[s]);
-
-    if (packed) {
-      if (lane < 8) stage[lane] = L[lane];
-      if (lane == 8) stage[8 + total_bits / 32] = (uint32_t)osize;
-      int p0 = 8 * kSub;
-#pragma unroll
-      for (int s = 0; s < kSub; s++) {
-        const int ln = sub_length(L, s);
-        if (ln) {
-          W v[PL];
-          lane_values(raw[s], v);
-#pragma unroll
-          for (int k = 0; k < PL; k++) v[k] = to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
-          or_field(stage, p0 + lane * PL * ln, lane_field(v, ln));
-        }
-        p0 += ln * Word<W>::PerSub;
-      }
-    } else {
-      // raw chunk: the original bytes
-#pragma unroll
-      for (int s = 0; s < kSub; s++) ((uint2*)stage)[lane + s * (kSubBytes / 8)] = raw[s];
-    }
-
-    if (lane == 0) {
-      const long long g = c / kGroup;
-      const long long members = min((long long)kGroup, nchunks - g * kGroup);
-      if ((long long)(grp_old >> 32) + 1 == members)
-        flag_max(st.group + g, (g == 0 ? kInc : kAgg) | ((grp_old & 0xffffffffull) + (unsigned)csize));
-    }
-    const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
-    if (lane == 0 && c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)csize);
-    __syncthreads();   // packed payload complete
-
-    wave_store_unaligned(a.out + data_base + (long long)excl, stage, csize);
-    if (lane == 0) {
-      if (c == 0) *(long long*)a.out = a.insize;
-      if (c == nchunks - 1) *a.outsize = data_base + (long long)excl + csize;
-    }
-    __syncthreads();   // stage read by the stores before the next chunk clears it
-  }
+  const ScanState st = scan_state(a.ws, nchunks);
+  for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) encode_chunk_wave<W, LOSSY>(a, st, c, nchunks, stage);
 }
 
-// Validity of a stream header against the caller's limits (uniform).
-__device__ __forceinline__ bool header_ok(const DecodeArgs& a, long long hdr, long long* nchunks, uint64_t* why)
-{
-  if (hdr < 0) { *why = kStCorrupt; return false; }
-  if (a.cap >= 0 && hdr > a.cap) { *why = kStCapacity; return false; }
-  *nchunks = (hdr + kChunk - 1) / kChunk;
-  if (a.inlen >= 0 && 8 + 2 * *nchunks > a.inlen) { *why = kStCorrupt; return false; }
-  return true;
-}
-
-// Exclusive prefix sums of the uint16 size table into the workspace (one
-// int64 per chunk plus the total), by one workgroup of 1024 threads: each
-// thread takes 32 consecutive entries (8-byte loads), so one pass covers
-// 32768 chunks (512 MiB of f32 input) and larger tables loop.  Also writes
-// *decsize.
-constexpr int kScanThreads = 1024;
-constexpr int kScanPer = 32;
 __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
 {
   __shared__ long long wsum[kScanThreads / 64];
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const long long hdr = uni(*(const long long*)a.in);
-  long long nchunks = 0;
-  uint64_t why = 0;
-  if (!header_ok(a, hdr, &nchunks, &why)) {
-    if (t == 0) raise_status(a.ws, why);
-    return;
-  }
-  if (t == 0) *a.decsize = hdr;
-  const uint16_t* const tab = (const uint16_t*)(a.in + 8);
-  long long* const offs = (long long*)(a.ws + kWsFlags);
-  long long carry = 0;
-  for (long long base = 0; base < nchunks; base += (long long)kScanThreads * kScanPer) {
-    const long long i0 = base + (long long)t * kScanPer;
-    uint32_t v[kScanPer];
-    if (i0 + kScanPer <= nchunks) {
-      const unsigned long long* p = (const unsigned long long*)(tab + i0);   // 8-byte aligned
-#pragma unroll
-      for (int k = 0; k < kScanPer / 4; k++) {
-        const unsigned long long w = p[k];
-        v[4 * k] = (uint32_t)(w & 0xffff);
-        v[4 * k + 1] = (uint32_t)((w >> 16) & 0xffff);
-        v[4 * k + 2] = (uint32_t)((w >> 32) & 0xffff);
-        v[4 * k + 3] = (uint32_t)(w >> 48);
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < kScanPer; k++) v[k] = (i0 + k < nchunks) ? (uint32_t)tab[i0 + k] : 0u;
-    }
-    uint32_t local = 0;
-#pragma unroll
-    for (int k = 0; k < kScanPer; k++) local += v[k];
-    uint32_t inc = local;   // inclusive wave scan of the per-thread sums
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += y;
-    }
-    if (lane == 63) wsum[wave] = inc;
-    __syncthreads();
-    long long before = 0, tile = 0;
-#pragma unroll
-    for (int w = 0; w < kScanThreads / 64; w++) {
-      const long long x = wsum[w];
-      if (w < wave) before += x;
-      tile += x;
-    }
-    long long run = carry + before + (long long)(inc - local);
-    if (i0 + kScanPer <= nchunks) {
-      longlong2* o = (longlong2*)(offs + i0);   // 16-byte aligned: i0 is a multiple of 32
-#pragma unroll
-      for (int k = 0; k < kScanPer / 2; k++) {
-        const long long r0 = run;
-        run += v[2 * k];
-        o[k] = make_longlong2(r0, run);
-        run += v[2 * k + 1];
-      }
-    } else {
-#pragma unroll
-      for (int k = 0; k < kScanPer; k++) {
-        if (i0 + k < nchunks) offs[i0 + k] = run;
-        run += v[k];
-      }
-    }
-    carry += tile;
-    __syncthreads();
-  }
-  if (t == 0) offs[nchunks] = carry;
+  scan_offsets(a, wsum);
 }
 
-// Chunk decode: no cross-workgroup dependency once the offsets are known,
-// so workgroups stride over chunks in any order.
+// Decoder: workgroups stride over the chunks in any order.
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 {
-  constexpr int PL = Word<W>::PerLane;
-  constexpr int TB = Word<W>::TB;
   __shared__ uint32_t stage[kStageBytes / 4];
-
-  const int t = threadIdx.x;
-  const int lane = t & 63;
-  const int wv = t >> 6;
   const long long hdr = uni(*(const long long*)a.in);
   long long nchunks = 0;
   uint64_t why = 0;
   if (!header_ok(a, hdr, &nchunks, &why)) return;   // reported by offsets_kernel
-  const long long data_base = 8 + 2 * nchunks;
-  const uint16_t* const sizes = (const uint16_t*)(a.in + 8);
-  const long long* const offs_tab = (const long long*)(a.ws + kWsFlags);
-  const W offs = (W)a.offs;
-  if (a.inlen >= 0 && data_base + uni(offs_tab[nchunks]) > a.inlen) {
-    if (blockIdx.x == 0 && t == 0) raise_status(a.ws, kStCorrupt);
-    return;
-  }
-
-  for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) {
-    const long long base = c * kChunk;
-    const int osize = (int)min((long long)kChunk, hdr - base);
-    const int csize = uni((int)sizes[c]);
-    const long long excl = uni(offs_tab[c]);
-    const bool bad = csize > osize || (csize < osize && csize < kSub + 2);
-
-    if (!bad) {
-      // aligned cover of the payload: payload byte i sits at stage byte mis + i
-      const uint8_t* const src = a.in + data_base + excl;
-      const int mis = (int)((uintptr_t)src & 15);
-      const uint4* const cover = (const uint4*)(src - mis);
-      const int nblk = (mis + csize + 15) >> 4;
-      for (int j = t; j < nblk; j += kThreads) ((uint4*)stage)[j] = cover[j];
-      __syncthreads();
-
-      uint8_t* const dst = a.out + base;
-      if (csize == osize) {
-#pragma unroll
-        for (int r = 0; r < kRounds; r++) {
-          const int off = r * kThreads * 16 + t * 16;
-          if (off < osize) store_slice(dst, off, osize, lds_read16(stage, mis + off));
-        }
-      } else {
-        // 32 bit-length bytes, realigned from the cover
-        uint32_t L[8];
-        const int q = mis >> 2;
-        const uint32_t sh = (uint32_t)(mis & 3) * 8u;
-        bool over = false;
-        uint32_t nb = 0;
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-          L[i] = uni(__builtin_amdgcn_alignbit(stage[q + i + 1], stage[q + i], sh));
-          over |= any_byte_above(L[i], (uint32_t)TB);
-          nb += byte_sum(L[i]);
-        }
-        const uint8_t* sb = (const uint8_t*)stage;
-        const int tr = uni((int)(int16_t)((uint16_t)sb[mis + csize - 2] | ((uint16_t)sb[mis + csize - 1] << 8)));
-        const bool corrupt = over || (long long)(kSub + 2) + (long long)nb * (Word<W>::PerSub / 8) != csize || tr != osize;
-        if (corrupt) {
-          if (t == 0) raise_status(a.ws, kStCorrupt);
-        } else {
-          int p0 = 8 * (mis + kSub) + sub_bit_offset<W>(L, wv * kWaveRounds);
-          const int woff = wv * (kWaveRounds * kSubBytes) + lane * 8;
-#pragma unroll
-          for (int r = 0; r < kWaveRounds; r++) {
-            const int ln = sub_length(L, wv * kWaveRounds + r);
-            const uint64_t f = read_field(stage, p0 + lane * PL * ln);
-            p0 += ln * Word<W>::PerSub;
-            W v[PL];
-            if constexpr (PL == 2) {
-              const uint32_t m = ln >= 32 ? 0xffffffffu : ((1u << ln) - 1u);
-              v[0] = from_code<W, LOSSY>((W)f & m, a.thr_e, offs);
-              v[1] = from_code<W, LOSSY>((W)(f >> ln) & m, a.thr_e, offs);
-            } else {
-              const uint64_t m = ln >= 64 ? ~0ull : ((1ull << ln) - 1ull);
-              v[0] = from_code<W, LOSSY>((W)(f & m), a.thr_e, offs);
-            }
-            store_lane(dst, woff + r * kSubBytes, osize, values_lane(v));
-          }
-        }
-      }
-    } else if (t == 0) {
-      raise_status(a.ws, kStCorrupt);
-    }
-    __syncthreads();
-  }
+  for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) decode_chunk<W, LOSSY>(a, hdr, nchunks, c, stage);
 }
 
 // ---- launchers ----

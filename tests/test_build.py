@@ -13,7 +13,7 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-SRC = os.path.join(ROOT, "sleek_amd", "csrc", "sleek_kernels.hip")
+SRCS = [os.path.join(ROOT, "sleek_amd", "csrc", f) for f in ("sleek_kernels.hip", "sleek_batch.hip")]
 
 
 def kernel_barriers(asm):
@@ -45,19 +45,28 @@ def asm(tmp_path_factory):
     if not os.path.exists(hipcc):
         pytest.skip("hipcc not available")
     d = tmp_path_factory.mktemp("asm")
-    out = d / "k.s"
-    subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-S",
-                    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", SRC, "-o", str(out)],
-                   check=True, capture_output=True)
-    return out.read_text()
+    text = ""
+    for i, src in enumerate(SRCS):
+        out = d / f"k{i}.s"
+        subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "--cuda-device-only", "-S",
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", src, "-o", str(out)],
+                       check=True, capture_output=True)
+        text += out.read_text()
+    return text
 
 
 def test_barriers_in_single_outer_loop(asm):
     b = kernel_barriers(asm)
     enc = [k for k in b if "encode_kernel" in k]
-    dec = [k for k in b if "decode_kernel" in k]
+    dec = [k for k in b if "decode_kernel" in k and "batch" not in k]
     assert len(enc) == 8 and len(dec) == 4
     for k in enc:
         assert b[k] == [1, 1, 1], (k, b[k])      # bit lengths, payload ready, stage reuse
     for k in dec:
         assert b[k] == [1, 1], (k, b[k])          # payload cover loaded, stage reuse
+    # the batched decoder inlines the chunk decoder once per (dtype, mode)
+    bdec = [k for k in b if "batch_decode_kernel" in k]
+    assert len(bdec) == 1 and b[bdec[0]] == [1] * 8, b.get(bdec[0] if bdec else None)
+    # single-wave workgroups (one-wave encoder, batched encoder) need no barrier instruction
+    assert not [k for k in b if "wave_kernel" in k or "batch_encode" in k]
+    assert all(d == 1 for v in b.values() for d in v), b

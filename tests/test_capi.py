@@ -96,3 +96,79 @@ def test_argument_checks_before_any_device_work():
     assert lib.sleek_gpu_decompress_async(4, 0, 0, 0, 8, -1, 20, 0, 8, 8, 1024, 0, None) == _lib.EALIGN
     assert lib.sleek_gpu_decompress_async(4, 0, 0, 0, 8, -1, 16, 16384 * 100, 8, 8, 64, 0, None) == _lib.EWORKSPACE
     assert lib.sleek_gpu_workspace_status(None, None) == _lib.EINVAL
+
+
+def _items(specs, decompress):
+    """Host batch items with fake aligned device pointers (the plan never dereferences them)."""
+    from sleek_amd.batch import ITEM
+
+    items = np.zeros(len(specs), dtype=ITEM)
+    for i, (nbytes, eb_, lossy) in enumerate(specs):
+        base = 0x10000000 + i * 0x1000000
+        items[i]["src"], items[i]["dst"], items[i]["d_size"] = base, base + 0x800000, 0x2000 + 8 * i
+        items[i]["elem_bytes"], items[i]["lossy"] = eb_, lossy
+        if decompress:
+            items[i]["src_bytes"], items[i]["dst_bytes"] = -1, nbytes
+        else:
+            items[i]["src_bytes"] = nbytes
+    return items
+
+
+def test_batch_item_layout_matches_header():
+    from sleek_amd.batch import ITEM
+
+    hdr = open(os.path.join(ROOT, "include", "sleek_gpu.h")).read()
+    assert "} sleek_gpu_batch_item;     /* 96 bytes */" in hdr
+    assert ITEM.itemsize == 96
+    assert [ITEM.fields[f][1] for f in ("src", "offs", "eb_e", "elem_bytes", "lossy", "tile0", "ws_words")] == \
+        [0, 40, 48, 56, 60, 64, 88]
+
+
+@pytest.mark.parametrize("decompress", [0, 1])
+def test_batch_plan_layout(decompress):
+    from sleek_amd.batch import plan
+
+    specs = [(0, 4, 0), (4, 4, 1), (16384, 8, 0), (16388, 4, 0), (400000, 8, 1), (16384 * 65, 4, 1)]
+    items = _items(specs, decompress)
+    tiles, wsb = plan(items, decompress)
+    even = lambda w: (w + 1) & ~1
+    t = 0
+    word = 16 + even(len(specs)) + even((sum(max(1, -(-b // 16384)) for b, _, _ in specs) + 1) // 2)
+    for it, (b, _, _) in zip(items, specs):
+        ch = -(-b // 16384)
+        assert it["tile0"] == t and it["tiles"] == max(1, ch)
+        assert it["ws_words"] == even(16 + ch + 2 * (-(-ch // 64)))
+        assert it["ws_word"] == word and it["ws_word"] % 2 == 0
+        t += max(1, ch)
+        word += it["ws_words"]
+    assert tiles == t and wsb == word * 8
+
+
+def test_batch_plan_rejects_bad_items():
+    from sleek_amd import _lib
+    from sleek_amd.batch import plan
+
+    def rc_of(items, d):
+        try:
+            plan(items, d)
+            return 0
+        except _lib.SleekError as e:
+            return e.status
+
+    it = _items([(100, 4, 0)], 0)
+    it[0]["src_bytes"] = 102            # not a multiple of the element size
+    assert rc_of(it, 0) == _lib.EINVAL
+    it = _items([(100, 6, 0)], 0)       # element size
+    assert rc_of(it, 0) == _lib.EINVAL
+    it = _items([(100, 4, 0)], 0)
+    it[0]["src"] += 8                   # arrays need 16-byte alignment
+    assert rc_of(it, 0) == _lib.EALIGN
+    it = _items([(100, 4, 0)], 1)
+    it[0]["dst_bytes"] = -1             # a capacity is required to decompress
+    assert rc_of(it, 1) == _lib.EINVAL
+    it = _items([(100, 4, 0)], 1)
+    it[0]["src_bytes"] = 7              # shorter than the header
+    assert rc_of(it, 1) == _lib.EINVAL
+    it = _items([(0, 4, 0)], 0)
+    it[0]["src"] = 0                    # an empty array may have no data pointer
+    assert rc_of(it, 0) == 0

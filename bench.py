@@ -40,6 +40,10 @@ def parse():
                    help="0 pipelined (default), 1 pipelined at the compiler's register choice, 2 one wave per chunk")
     p.add_argument("--encode-grid", type=int, default=0, help="encoder persistent grid (0 = automatic)")
     p.add_argument("--decode-grid", type=int, default=0, help="decoder grid (0 = one workgroup per chunk)")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    p.add_argument("--batch-arrays", type=int, default=4096)
+    p.add_argument("--batch-max-mb", type=float, default=64.0, help="largest array of the batched workload")
+    p.add_argument("--batch-steps", type=int, default=3)
     return p.parse_args()
 
 
@@ -76,6 +80,141 @@ def cpu_baseline(a_np, mb):
         "compress_GBps": round(sample.nbytes / tcm / 1e9, 3),
         "decompress_GBps": round(sample.nbytes / tdm / 1e9, 3),
     }
+
+
+def bench_batch(args, dev):
+    """The uneven batch: log-uniform 4 KiB .. 64 MiB arrays, 70 % float32, half
+    lossless and half lossy at {1e-2, 1e-3, 1e-4} x range.  One graph-captured
+    batched launch per direction against per-array plain launches (the async
+    C ABI in a loop on one stream, one reused workspace); both produce the
+    same bytes, which is checked.  Rates are aggregate input bytes / time."""
+    import ctypes
+
+    import torch
+
+    from sleek_amd import _lib, codec
+    from sleek_amd.batch import plan_compress, plan_decompress
+    from sleek_amd.datagen import array_field, batch_sizes, finite_range
+
+    lib = _lib.load()
+    t_gen = time.perf_counter()
+    specs = batch_sizes(args.batch_arrays, seed=args.seed + 77, hi=int(args.batch_max_mb * (1 << 20)))
+    rng = np.random.default_rng(args.seed + 78)
+    xs, ebs = [], []
+    for i, (nb, dt) in enumerate(specs):
+        x = array_field(nb, dt, seed=args.seed * 100003 + i, device=dev)
+        xs.append(x)
+        if rng.random() < 0.5:
+            ebs.append(None)
+        else:
+            rel = float(rng.choice([1e-2, 1e-3, 1e-4]))
+            r = finite_range(x)
+            ebs.append(float(np.dtype(dt).type(np.dtype(dt).type(r) * np.dtype(dt).type(rel))) if r > 0 else rel)
+    torch.cuda.synchronize()
+    t_gen = time.perf_counter() - t_gen
+    total = sum(x.numel() * x.element_size() for x in xs)
+    print(f"batched workload: {len(xs)} arrays, {total / 1e9:.2f} GB generated in {t_gen:.1f} s",
+          file=sys.stderr, flush=True)
+    stream = torch.cuda.current_stream()
+    h = stream.cuda_stream
+
+    def timed(fn, reps):
+        ts = []
+        for _ in range(reps):
+            a0, a1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a0.record(stream)
+            fn()
+            a1.record(stream)
+            torch.cuda.synchronize()
+            ts.append(a0.elapsed_time(a1) / 1e3)
+        return float(np.median(ts))
+
+    # batched, graph-captured
+    bc = plan_compress(xs, ebs)
+    gc = bc.capture()
+    gc.launch()
+    torch.cuda.synchronize()
+    assert not bc.status().any(), "batched compress status"
+    sizes = bc.sizes.cpu().numpy()[: len(xs)]
+    streams = [bc.outs[i][: int(sizes[i])] for i in range(len(xs))]
+    bd = plan_decompress(streams, [x.dtype for x in xs], ebs, [x.numel() for x in xs])
+    gd = bd.capture()
+    gd.launch()
+    torch.cuda.synchronize()
+    assert not bd.status().any(), "batched decompress status"
+    print("batched workload: graphs captured and checked", file=sys.stderr, flush=True)
+    t_bc = timed(gc.launch, args.batch_steps)
+    t_bd = timed(gd.launch, args.batch_steps)
+    t_bc_eager = timed(lambda: bc.launch(), args.batch_steps)
+
+    # per-array plain launches through the C ABI
+    maxb = max(x.numel() * x.element_size() for x in xs)
+    ws = codec.Workspace(maxb, dev)
+    pouts = [torch.empty_like(o) for o in bc.outs]
+    psz = torch.zeros(len(xs), dtype=torch.int64, device=dev)
+    pdec = [torch.empty_like(x) for x in xs]
+    pdsz = torch.zeros(len(xs), dtype=torch.int64, device=dev)
+    cargs, dargs = [], []
+    for i, (x, eb) in enumerate(zip(xs, ebs)):
+        eb_b = x.element_size()
+        e, t, o = (0, 0, 0) if eb is None else codec.error_bound(eb, x.dtype)
+        nb = x.numel() * eb_b
+        cargs.append((eb_b, int(eb is not None), e, t, o, x.data_ptr(), nb, pouts[i].data_ptr(),
+                      psz.data_ptr() + 8 * i, ws.ptr(), ws.nbytes, 0, h))
+        dargs.append((eb_b, int(eb is not None), t, o, streams[i].data_ptr(), int(sizes[i]), pdec[i].data_ptr(), nb,
+                      pdsz.data_ptr() + 8 * i, ws.ptr(), ws.nbytes, 0, h))
+    comp, decomp = lib.sleek_gpu_compress_async, lib.sleek_gpu_decompress_async
+
+    def plain_c():
+        for a in cargs:
+            comp(*a)
+
+    def plain_d():
+        for a in dargs:
+            decomp(*a)
+
+    for a in cargs:
+        _lib.check(comp(*a), "plain compress")
+    for a in dargs:
+        _lib.check(decomp(*a), "plain decompress")
+    torch.cuda.synchronize()
+    assert ws.status() == 0
+    print("batched workload: plain launches warmed up", file=sys.stderr, flush=True)
+    t_pc = timed(plain_c, args.batch_steps)
+    t_pd = timed(plain_d, args.batch_steps)
+
+    # the two forms wrote the same bytes
+    assert torch.equal(psz[: len(xs)], bc.sizes[: len(xs)]), "batched and plain stream sizes differ"
+    for i in range(len(xs)):
+        n = int(sizes[i])
+        assert torch.equal(pouts[i][:n], bc.outs[i][:n]), f"stream {i} differs"
+        assert torch.equal(pdec[i].view(torch.uint8), bd.outs[i].view(torch.uint8)), f"decoded array {i} differs"
+        if ebs[i] is None:
+            assert torch.equal(bd.outs[i].view(torch.uint8), xs[i].view(torch.uint8)), f"lossless array {i} differs"
+    gc.close()
+    gd.close()
+    enc_total = int(sizes.sum())
+    out = {
+        "arrays": len(xs), "input_bytes": total, "stream_bytes": enc_total,
+        "f32_arrays": sum(1 for x in xs if x.dtype == torch.float32),
+        "lossy_arrays": sum(1 for e in ebs if e is not None),
+        "sizes": f"log-uniform 4 KiB .. {args.batch_max_mb:g} MiB",
+        "tiles": bc.tiles,
+        "compress": {"batched_graph_ms": round(t_bc * 1e3, 3), "batched_eager_ms": round(t_bc_eager * 1e3, 3),
+                     "plain_ms": round(t_pc * 1e3, 3),
+                     "batched_GBps": round(total / t_bc / 1e9, 1), "plain_GBps": round(total / t_pc / 1e9, 1),
+                     "speedup": round(t_pc / t_bc, 3),
+                     "batched_hbm_frac": round((total + enc_total) / t_bc / HBM_PEAK, 4)},
+        "decompress": {"batched_graph_ms": round(t_bd * 1e3, 3), "plain_ms": round(t_pd * 1e3, 3),
+                       "batched_GBps": round(total / t_bd / 1e9, 1), "plain_GBps": round(total / t_pd / 1e9, 1),
+                       "speedup": round(t_pd / t_bd, 3),
+                       "batched_hbm_frac": round((total + enc_total) / t_bd / HBM_PEAK, 4)},
+        "generate_s": round(t_gen, 2),
+        "identical": True,
+    }
+    del xs, pouts, pdec, bc, bd, streams
+    torch.cuda.empty_cache()
+    return out
 
 
 def main():
@@ -205,8 +344,13 @@ def main():
                        "hbm_frac": round(2 * nbytes / mtm / HBM_PEAK, 4)},
         "wall_s": round(wall, 3),
     }
-    if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(x.cpu().numpy(), args.cpu_mb)
+    x_host = None if args.no_cpu else x.cpu().numpy()
+    del x, bufs, decs, y
+    torch.cuda.empty_cache()
+    if not args.no_batch:
+        out["batched"] = bench_batch(args, dev)
+    if x_host is not None:
+        out["cpu_baseline"] = cpu_baseline(x_host, args.cpu_mb)
     print(json.dumps(out), flush=True)
 
 

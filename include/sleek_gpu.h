@@ -108,6 +108,12 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
  * change speed only, never the bytes.  Applies to calls made after it
  * returns; SLEEK_EINVAL for an unknown variant or a negative size. */
 int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid);
+/* Look-back geometry of the encoders (no reference counterpart; the
+ * reference watches a fixed window of 32 chunks, sleek_common_gpu.h:220-253):
+ * group_chunks = chunks per look-back group, 8, 16, 32 or 64 (0 = default,
+ * 64); window_groups = groups watched by the first group-level poll, 1..64
+ * (0 = default, 4).  Speed only, never the bytes; applies to later calls. */
+int sleek_gpu_set_lookback(int group_chunks, int window_groups);
 
 /* Synchronises `stream`, then returns SLEEK_OK, SLEEK_ECORRUPT or
  * SLEEK_ECAPACITY from the workspace's status word. */

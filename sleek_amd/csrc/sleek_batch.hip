@@ -34,7 +34,7 @@ __device__ __forceinline__ int item_kind(const BatchItem* it)
 
 // Encoder: one wave per tile (encode_chunk_wave).
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void batch_encode_kernel(
-    const BatchItem* items, const uint32_t* tile_map, long long tiles, uint64_t* ws)
+    const BatchItem* items, const uint32_t* tile_map, long long tiles, uint64_t* ws, int lb_shift, int lb_window)
 {
   __shared__ uint32_t stage[kStageBytes / 4];
   for (long long t = blockIdx.x; t < tiles; t += gridDim.x) {
@@ -48,6 +48,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     a.eb_e = it->eb_e;
     a.thr_e = it->thr_e;
     a.offs = it->offs;
+    a.lb_shift = lb_shift;
+    a.lb_window = lb_window;
     const long long nchunks = (a.insize + kChunk - 1) / kChunk;
     const long long c = t - it->tile0;
     const int kind = item_kind(it);
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
         *a.outsize = 8;
       }
     } else {
-      const ScanState st = scan_state(a.ws, nchunks);
+      const ScanState st = scan_state(a.ws, nchunks, lb_shift);
       if (kind == 0) encode_chunk_wave<uint32_t, false>(a, st, c, nchunks, stage);
       else if (kind == 1) encode_chunk_wave<uint32_t, true>(a, st, c, nchunks, stage);
       else if (kind == 2) encode_chunk_wave<uint64_t, false>(a, st, c, nchunks, stage);
@@ -144,7 +146,7 @@ int batch_resident_grid(bool decompress)
 }
 
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        hipStream_t stream)
+                        int lb_shift, int lb_window, hipStream_t stream)
 {
   if (n <= 0) return hipSuccess;
   uint32_t* const map = (uint32_t*)(ws + batch_map_word(n));
@@ -156,7 +158,8 @@ hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long lon
     hipLaunchKernelGGL(batch_offsets_kernel, dim3(n), dim3(kScanThreads), 0, stream, items, ws);
     hipLaunchKernelGGL(batch_decode_kernel, dim3((unsigned)g), dim3(kThreads), 0, stream, items, map, tiles, ws);
   } else {
-    hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)g), dim3(64), 0, stream, items, map, tiles, ws);
+    hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)g), dim3(64), 0, stream, items, map, tiles, ws, lb_shift,
+                       lb_window);
   }
   return hipGetLastError();
 }

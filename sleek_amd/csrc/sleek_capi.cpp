@@ -29,12 +29,14 @@ int hip_fail(hipError_t e, const char* what)
 long long chunks_of(long long bytes) { return (bytes + SLEEK_CHUNK_BYTES - 1) / SLEEK_CHUNK_BYTES; }
 
 // workspace: 8 header words (word 1: status), then per chunk one word, and
-// per group of 64 chunks two words (look-back state when encoding; chunk
-// offsets and their total when decoding); 8 spare words, an even total
+// per look-back group (8 chunks at the smallest setting) two words (look-back
+// state when encoding; chunk offsets and their total when decoding); 8 spare
+// words, an even total
 long long ws_words_for(long long insize)
 {
   const long long ch = chunks_of(insize);
-  return sleek::even_words(16 + ch + 2 * ((ch + 63) / 64));
+  const long long gmin = 1ll << sleek::kGroupShiftMin;
+  return sleek::even_words(16 + ch + 2 * ((ch + gmin - 1) / gmin));
 }
 
 long long ws_bytes_for(long long insize) { return ws_words_for(insize) * 8; }
@@ -43,6 +45,8 @@ long long ws_bytes_for(long long insize) { return ws_words_for(insize) * 8; }
 int g_encode_variant = sleek::kEncodePipelined;
 int g_encode_grid = 0;
 int g_decode_grid = 0;
+int g_lb_shift = 6;    // look-back groups of 64 chunks
+int g_lb_window = 4;   // initial look-back window, in groups
 
 // Per-device scratch of the synchronous entry points (grow-only).
 struct DeviceScratch {
@@ -109,6 +113,8 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   a.eb_e = eb_e;
   a.thr_e = thr_e;
   a.offs = offs;
+  a.lb_shift = g_lb_shift;
+  a.lb_window = g_lb_window;
   const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
   e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_variant, stream);
   if (e != hipSuccess) return hip_fail(e, "encode launch");
@@ -320,6 +326,21 @@ int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid)
   return SLEEK_OK;
 }
 
+int sleek_gpu_set_lookback(int group_chunks, int window_groups)
+{
+  if (group_chunks < 0 || window_groups < 0 || window_groups > 64) return SLEEK_EINVAL;
+  int shift = sleek::kGroupShiftMax;
+  if (group_chunks > 0) {
+    for (shift = sleek::kGroupShiftMin; shift <= sleek::kGroupShiftMax && (1 << shift) != group_chunks; shift++) {
+    }
+    if (shift > sleek::kGroupShiftMax) return SLEEK_EINVAL;
+  }
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_lb_shift = shift;
+  g_lb_window = window_groups > 0 ? window_groups : 4;
+  return SLEEK_OK;
+}
+
 long long sleek_gpu_workspace_size(long long max_insize)
 {
   if (max_insize < 0) return SLEEK_EINVAL;
@@ -404,7 +425,7 @@ static int batch_async(bool decompress, const sleek_gpu_batch_item* d_items, int
   if (!aligned(ws, 16) || !aligned(d_items, 8)) return SLEEK_EALIGN;
   if (ws_bytes < (sleek::batch_map_word(n) + (tiles + 1) / 2) * 8) return SLEEK_EWORKSPACE;
   const hipError_t e = sleek::batch_launch(decompress, (const sleek::BatchItem*)d_items, n, tiles, (uint64_t*)ws, grid,
-                                           (hipStream_t)stream);
+                                           g_lb_shift, g_lb_window, (hipStream_t)stream);
   return e == hipSuccess ? SLEEK_OK : hip_fail(e, "batch launch");
 }
 

@@ -97,8 +97,8 @@ template <typename W, bool LOSSY>
 __device__ uint64_t wave_group_size(const EncodeArgs& a, const ScanState& s, long long gj, long long nchunks)
 {
   const int lane = threadIdx.x & 63;
-  const long long first = gj * kGroup;
-  const int members = (int)min((long long)kGroup, nchunks - first);
+  const long long first = gj << s.gshift;
+  const int members = (int)group_members(s, gj, nchunks);
   uint64_t f = (lane < members) ? flag_load(s.chunk + first + lane) : kAgg;
   uint64_t mine = ((f >> 62) == 1) ? (f & kValMask) : 0;
   uint64_t todo = __ballot(lane < members && (f >> 62) != 1);
@@ -118,9 +118,10 @@ constexpr uint64_t kHelpTicks = 10000;   // 100 us
 // Two-level decoupled look-back, run by one whole wave; returns the sum of
 // the compressed sizes of chunks 0..c-1 in every lane.  First the preceding
 // chunks of c's own group are watched (one lane each); once they are all
-// published, a window of earlier groups is watched, starting 4 groups wide
-// and doubling to 64 as it slides back, so the inclusive frontier can
-// advance up to 4096 chunks per round trip while a poll reads few words.
+// published, a window of earlier groups is watched, starting a.lb_window
+// groups wide and doubling up to 64 as it slides back, so the inclusive
+// frontier can advance up to 64 groups per round trip while a poll reads
+// few words.  Groups hold 2^gshift chunks (8..64).
 // Polls back off exponentially: the words live at device scope, and many
 // waiting waves polling at full rate would compete with the data traffic.
 // Workgroups take chunks in launch order, so the words waited on are
@@ -131,9 +132,9 @@ template <typename W, bool LOSSY>
 __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long c, long long nchunks)
 {
   const int lane = threadIdx.x & 63;
-  const int q = (int)(c % kGroup);                 // chunks of the group before c
-  long long gend = c / kGroup;                      // group window: groups gend-gwin .. gend-1
-  int gwin = 4;
+  const int q = (int)(c & ((1ll << s.gshift) - 1));   // chunks of the group before c
+  long long gend = c >> s.gshift;                       // group window: groups gend-gwin .. gend-1
+  int gwin = a.lb_window;
   bool have_partial = (q == 0);
   uint64_t partial = 0, gacc = 0;
   uint64_t since = __builtin_amdgcn_s_memrealtime();
@@ -174,7 +175,7 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
       } else if (rdy == ~0ull) {
         gacc += wave_sum(fg & kValMask);
         gend -= gwin;
-        if (gwin < 64) gwin *= 2;
+        gwin = min(2 * gwin, 64);
         progressed = true;
       }
       if (!progressed && __builtin_amdgcn_s_memrealtime() - since > kHelpTicks) {
@@ -258,7 +259,7 @@ __device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const Sca
   unsigned long long grp_old = 0;
   if (lane == 0) {
     flag_max(st.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
-    grp_old = __hip_atomic_fetch_add(st.count + c / kGroup, (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
+    grp_old = __hip_atomic_fetch_add(st.count + (c >> st.gshift), (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
     sizes[c] = (uint16_t)csize;
   }
@@ -289,9 +290,8 @@ __device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const Sca
   }
 
   if (lane == 0) {
-    const long long g = c / kGroup;
-    const long long members = min((long long)kGroup, nchunks - g * kGroup);
-    if ((long long)(grp_old >> 32) + 1 == members)
+    const long long g = c >> st.gshift;
+    if ((long long)(grp_old >> 32) + 1 == group_members(st, g, nchunks))
       flag_max(st.group + g, (g == 0 ? kInc : kAgg) | ((grp_old & 0xffffffffull) + (unsigned)csize));
   }
   const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);

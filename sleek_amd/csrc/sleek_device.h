@@ -26,15 +26,14 @@ constexpr int kStageBytes = kChunk + 64;            // LDS staging buffer (paylo
 
 // ---- scan state words ----
 // Workspace layout (uint64 words): [1] status bits, then (from
-// word 8) one look-back word per chunk, one per group of kGroup chunks, and
-// one completion counter per group.  A look-back word holds a byte count in
+// word 8) one look-back word per chunk, one per group of 2^gshift chunks
+// (8..64, a launch parameter), and one completion counter per group.  A look-back word holds a byte count in
 // bits 61..0 and its kind in bits 63..62: 0 = not published, 1 = aggregate
 // (the chunk's or group's own compressed size), 2 = inclusive (stream offset
 // just past it).  Kinds only ever increase, so inclusive words are written
 // with an atomic max and are never downgraded.
 constexpr int kWsStatus = 1;
 constexpr int kWsFlags = 8;
-constexpr int kGroup = 64;
 constexpr uint64_t kAgg = 1ull << 62;
 constexpr uint64_t kInc = 2ull << 62;
 constexpr uint64_t kValMask = (1ull << 62) - 1;
@@ -318,18 +317,26 @@ __device__ __forceinline__ void flag_max(uint64_t* p, uint64_t v)
 // Scan state of one encode call inside its workspace.
 struct ScanState {
   uint64_t* chunk;               // one word per chunk
-  uint64_t* group;               // one word per group of kGroup chunks
+  uint64_t* group;               // one word per group of 2^gshift chunks
   unsigned long long* count;     // per group: (chunks reported << 32) | bytes
+  int gshift;
 };
 
-__device__ __forceinline__ ScanState scan_state(uint64_t* ws, long long nchunks)
+__device__ __forceinline__ ScanState scan_state(uint64_t* ws, long long nchunks, int gshift)
 {
-  const long long ngroups = (nchunks + kGroup - 1) / kGroup;
+  const long long ngroups = (nchunks + (1ll << gshift) - 1) >> gshift;
   ScanState s;
   s.chunk = ws + kWsFlags;
   s.group = s.chunk + nchunks;
   s.count = (unsigned long long*)(s.group + ngroups);
+  s.gshift = gshift;
   return s;
+}
+
+// chunks of group g (the last group may be short)
+__device__ __forceinline__ long long group_members(const ScanState& s, long long g, long long nchunks)
+{
+  return min(1ll << s.gshift, nchunks - (g << s.gshift));
 }
 
 // Publish chunk c's compressed size: its own aggregate word, and the group's
@@ -338,11 +345,10 @@ __device__ __forceinline__ ScanState scan_state(uint64_t* ws, long long nchunks)
 __device__ __forceinline__ void publish_size(const ScanState& s, long long c, long long nchunks, uint32_t csize)
 {
   flag_max(s.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
-  const long long g = c / kGroup;
+  const long long g = c >> s.gshift;
   const unsigned long long old = __hip_atomic_fetch_add(s.count + g, (1ull << 32) | csize, __ATOMIC_RELAXED,
                                                         __HIP_MEMORY_SCOPE_AGENT);
-  const long long members = min((long long)kGroup, nchunks - g * kGroup);
-  if ((long long)(old >> 32) + 1 == members)
+  if ((long long)(old >> 32) + 1 == group_members(s, g, nchunks))
     flag_max(s.group + g, (g == 0 ? kInc : kAgg) | ((old & 0xffffffffull) + csize));
 }
 
@@ -350,8 +356,8 @@ __device__ __forceinline__ void publish_size(const ScanState& s, long long c, lo
 __device__ __forceinline__ void publish_inclusive(const ScanState& s, long long c, long long nchunks, uint64_t incl)
 {
   flag_max(s.chunk + c, kInc | incl);
-  const long long g = c / kGroup;
-  if ((c % kGroup) == kGroup - 1 || c == nchunks - 1) flag_max(s.group + g, kInc | incl);
+  const long long g = c >> s.gshift;
+  if (((c + 1) & ((1ll << s.gshift) - 1)) == 0 || c == nchunks - 1) flag_max(s.group + g, kInc | incl);
 }
 
 // 16 bytes starting at an arbitrary byte offset of a dword-aligned LDS buffer

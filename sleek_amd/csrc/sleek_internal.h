@@ -6,6 +6,10 @@
 
 namespace sleek {
 
+// look-back groups of 2^shift chunks: 8 .. 64 (one lane per chunk)
+constexpr int kGroupShiftMin = 3;
+constexpr int kGroupShiftMax = 6;
+
 struct EncodeArgs {
   const uint8_t* in;      // input array (16-byte aligned)
   long long insize;       // bytes
@@ -14,6 +18,8 @@ struct EncodeArgs {
   uint64_t* ws;           // zeroed workspace: status and look-back words
   int eb_e, thr_e;        // lossy parameters (ignored for lossless)
   unsigned long long offs;
+  int lb_shift;           // look-back groups of 2^lb_shift chunks (3..6)
+  int lb_window;          // initial look-back window in groups (1..64)
 };
 
 struct DecodeArgs {
@@ -70,7 +76,7 @@ inline long long even_words(long long w) { return (w + 1) & ~1ll; }
 inline long long batch_map_word(int n) { return kBatchHead + even_words(n); }
 
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        hipStream_t stream);
+                        int lb_shift, int lb_window, hipStream_t stream);
 hipError_t batch_status_launch(const BatchItem* items, int n, uint64_t* ws, hipStream_t stream);
 int batch_resident_grid(bool decompress);
 

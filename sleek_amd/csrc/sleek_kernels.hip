@@ -43,7 +43,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
   const long long data_base = 8 + 2 * nchunks;
   uint16_t* const sizes = (uint16_t*)(a.out + 8);
-  const ScanState st = scan_state(a.ws, nchunks);
+  const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
   const W offs = (W)a.offs;
   const int woff = wv * (kWaveRounds * kSubBytes) + lane * 8;   // this lane's byte in round 0
 
@@ -126,7 +126,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
       unsigned long long grp_old = 0;
       if (t == 0) {
         flag_max(st.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
-        grp_old = __hip_atomic_fetch_add(st.count + c / kGroup, (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
+        grp_old = __hip_atomic_fetch_add(st.count + (c >> st.gshift), (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
                                          __HIP_MEMORY_SCOPE_AGENT);
         sizes[c] = (uint16_t)csize;
       }
@@ -162,9 +162,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 
       if (t == 0) {
         // this chunk completes its group: publish the group's aggregate
-        const long long g = c / kGroup;
-        const long long members = min((long long)kGroup, nchunks - g * kGroup);
-        if ((long long)(grp_old >> 32) + 1 == members)
+        const long long g = c >> st.gshift;
+        if ((long long)(grp_old >> 32) + 1 == group_members(st, g, nchunks))
           flag_max(st.group + g, (g == 0 ? kInc : kAgg) | ((grp_old & 0xffffffffull) + (unsigned)csize));
       }
     }
@@ -211,7 +210,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
     }
     return;
   }
-  const ScanState st = scan_state(a.ws, nchunks);
+  const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
   for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) encode_chunk_wave<W, LOSSY>(a, st, c, nchunks, stage);
 }
 

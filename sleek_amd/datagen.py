@@ -20,7 +20,7 @@ import math
 
 import numpy as np
 
-__all__ = ["exact_field", "sinusoid_field", "inject_specials", "config_field", "batch_sizes"]
+__all__ = ["exact_field", "sinusoid_field", "inject_specials", "config_field", "batch_sizes", "array_field"]
 
 
 def exact_field(n, seed, dtype=np.float32, noise=1e-3, scale=10.0, offset=0.0):
@@ -194,3 +194,25 @@ def batch_sizes(n_arrays, seed, lo=4096, hi=64 << 20, f32_frac=0.7):
         es = 4 if f else 8
         out.append((max(es, int(s) // es * es), np.float32 if f else np.float64))
     return out
+
+
+def array_field(nbytes, dtype, seed, device=None):
+    """One array of the batched and serving workloads: a float32 array is
+    built like the 256^3 field (8 sinusoids + N(0, 1e-3)), a float64 one like
+    the 384^3 field (12 sinusoids + lognormal noise, 0.01 % specials), on the
+    smallest cube holding nbytes / itemsize values, flattened and cut."""
+    it = np.dtype(dtype).itemsize
+    n = nbytes // it
+    side = max(1, int(math.ceil(n ** (1.0 / 3.0) - 1e-9)))
+    while side ** 3 < n:
+        side += 1
+    if np.dtype(dtype) == np.float32:
+        f = sinusoid_field((side, side, side), seed, np.float32, 8, device=device)
+    else:
+        f = sinusoid_field((side, side, side), seed, np.float64, 12, noise="lognormal", noise_sigma=2.0,
+                           noise_mu=-10.0, device=device)
+    f = f.reshape(-1)[:n]
+    f = f.clone() if device is not None else f.copy()
+    if np.dtype(dtype) == np.float64:
+        inject_specials(f, 1e-4, seed)
+    return f

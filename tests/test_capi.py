@@ -78,6 +78,10 @@ def test_sizes_and_messages():
     assert lib.sleek_gpu_set_tuning(3, 0, 0) == _lib.EINVAL
     assert lib.sleek_gpu_set_tuning(0, -1, 0) == _lib.EINVAL
     assert lib.sleek_gpu_set_tuning(0, 0, 0) == 0
+    for g, w in ((12, 4), (4, 4), (128, 4), (64, 65), (-1, 0), (8, -1)):
+        assert lib.sleek_gpu_set_lookback(g, w) == _lib.EINVAL, (g, w)
+    for g, w in ((8, 1), (16, 64), (32, 7), (0, 0)):
+        assert lib.sleek_gpu_set_lookback(g, w) == 0, (g, w)
 
 
 def test_argument_checks_before_any_device_work():
@@ -137,7 +141,7 @@ def test_batch_plan_layout(decompress):
     for it, (b, _, _) in zip(items, specs):
         ch = -(-b // 16384)
         assert it["tile0"] == t and it["tiles"] == max(1, ch)
-        assert it["ws_words"] == even(16 + ch + 2 * (-(-ch // 64)))
+        assert it["ws_words"] == even(16 + ch + 2 * (-(-ch // 8)))   # groups as small as 8 chunks
         assert it["ws_word"] == word and it["ws_word"] % 2 == 0
         t += max(1, ch)
         word += it["ws_words"]

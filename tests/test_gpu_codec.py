@@ -129,6 +129,30 @@ def test_encoder_variant_invariance(gpu, dt, variant):
         lib.sleek_gpu_set_tuning(0, 0, 0)
 
 
+@pytest.mark.parametrize("variant", [0, 2])
+def test_lookback_geometry_invariance(gpu, variant):
+    """Look-back group sizes 8..64 and initial windows 1..64 (sleek_gpu_set_lookback)
+    change no byte, at the automatic grid and at a 3-workgroup grid."""
+    say(f"look-back geometry, encoder variant={variant}")
+    from sleek_amd import _lib
+    from sleek_amd.datagen import exact_field
+
+    lib = _lib.load()
+    a = exact_field(16384 // 4 * 700 + 33, 9, np.float32)   # 701 chunks: 11 groups of 64, 88 of 8
+    refs = {eb: po.compress(a, eb) for eb in (None, 1e-3)}
+    try:
+        for group, window in ((8, 1), (16, 7), (32, 64), (64, 4), (8, 64)):
+            _lib.check(lib.sleek_gpu_set_lookback(group, window), "lookback")
+            for grid in (0, 3):
+                _lib.check(lib.sleek_gpu_set_tuning(variant, grid, 0), "tuning")
+                for eb, ref in refs.items():
+                    s, _ = gpu_roundtrip(a, eb)
+                    assert s.tobytes() == ref.tobytes(), (group, window, grid, eb)
+    finally:
+        lib.sleek_gpu_set_lookback(0, 0)
+        lib.sleek_gpu_set_tuning(0, 0, 0)
+
+
 @pytest.mark.parametrize("config", [1, 2])
 def test_config_fields_scaled(gpu, config):
     """The benchmark recipes (sinusoids, ramp, lognormal noise, specials) at 1/4 scale."""

@@ -44,6 +44,11 @@ def parse():
     p.add_argument("--batch-arrays", type=int, default=4096)
     p.add_argument("--batch-max-mb", type=float, default=64.0, help="largest array of the batched workload")
     p.add_argument("--batch-steps", type=int, default=3)
+    p.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    p.add_argument("--serve-requests", type=int, default=2000)
+    p.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests/s")
+    p.add_argument("--serve-max-mb", type=float, default=256.0)
+    p.add_argument("--serve-budget-gb", type=float, default=96.0)
     return p.parse_args()
 
 
@@ -217,6 +222,102 @@ def bench_batch(args, dev):
     return out
 
 
+def bench_serve(args, dev):
+    """Serving: Poisson arrivals, request sizes log-uniform 64 KiB .. 256 MiB,
+    half compress and half decompress, 70 % float32, half lossless and half
+    lossy at {1e-2, 1e-3, 1e-4} x range, admission under a device-memory
+    budget, continuous batching over 4 streams.  Inputs are resident on the
+    device: compress requests read prefixes of pooled fields; decompress
+    requests read streams prepared before the run.  A sample of results is
+    checked against the same requests served alone."""
+    import torch
+
+    from sleek_amd import codec
+    from sleek_amd.batch import plan_compress
+    from sleek_amd.datagen import array_field, finite_range
+    from sleek_amd.serve import COMPRESS, DECOMPRESS, GpuEngine, Request, Server, poisson_arrivals, summarize
+
+    rng = np.random.default_rng(args.seed + 91)
+    hi = int(args.serve_max_mb * (1 << 20))
+    pools = {}
+    for dt in (np.float32, np.float64):
+        pools[dt] = []
+        for k in range(3):
+            f = array_field(hi, dt, seed=args.seed * 7919 + k + (100 if dt == np.float64 else 0), device=dev)
+            pools[dt].append((f, finite_range(f)))
+    n = args.serve_requests
+    arrivals = poisson_arrivals(n, args.serve_rate, args.seed + 92)
+    reqs = []
+    pre_x, pre_eb, pre_idx = [], [], []
+    for i in range(n):
+        dt = np.float32 if rng.random() < 0.7 else np.float64
+        it = np.dtype(dt).itemsize
+        nb = int(np.exp(rng.uniform(np.log(64 << 10), np.log(hi))))
+        m = max(1, nb // it)
+        f, r = pools[dt][int(rng.integers(0, 3))]
+        x = f[:m]
+        eb = None
+        if rng.random() < 0.5:
+            rel = float(rng.choice([1e-2, 1e-3, 1e-4]))
+            eb = float(np.dtype(dt).type(np.dtype(dt).type(r) * np.dtype(dt).type(rel)))
+        if rng.random() < 0.5:
+            reqs.append(Request(i, COMPRESS, x, dt, eb, arrival=float(arrivals[i])))
+        else:
+            reqs.append(Request(i, DECOMPRESS, None, dt, eb, numel=m, arrival=float(arrivals[i])))
+            pre_x.append(x)
+            pre_eb.append(eb)
+            pre_idx.append(i)
+    # the decompress requests' streams, prepared with one batched compress
+    pb = plan_compress(pre_x, pre_eb).launch()
+    torch.cuda.synchronize()
+    psz = pb.sizes.cpu().numpy()
+    for j, i in enumerate(pre_idx):
+        reqs[i].src = pb.outs[j][: int(psz[j])]
+    print(f"serving workload: {n} requests prepared", file=sys.stderr, flush=True)
+
+    sample = set(range(0, n, max(1, n // 24)))
+    keep = {}
+
+    def on_done(r):
+        if r.rid in sample:
+            keep[r.rid] = r.out
+        r.out = None
+
+    def serve_once(rs, engine):
+        for r in rs:
+            r.admitted = r.done = float("nan")
+        srv = Server(engine, budget_bytes=args.serve_budget_gb * 1e9, max_inflight=4, on_done=on_done)
+        srv.run(rs)
+        torch.cuda.synchronize()
+        return srv
+
+    engine = GpuEngine(dev, streams=4)
+    # warm-up pass over the first requests (allocator, pinned host buffers)
+    warm = [Request(r.rid, r.kind, r.src, r.dtype, r.eb, r.numel, r.arrival) for r in reqs[: min(64, n)]]
+    serve_once(warm, engine)
+    keep.clear()
+    srv = serve_once(reqs, engine)
+    stats = summarize(reqs, srv)
+    bad = [r.rid for r in reqs if r.status != 0]
+    assert not bad, f"requests with errors: {bad[:8]}"
+    for rid in sorted(keep):
+        r = reqs[rid]
+        if r.kind == COMPRESS:
+            alone, k = codec.compress(r.src, r.eb)
+            torch.cuda.synchronize()
+            assert torch.equal(keep[rid], alone[: int(k.item())]), f"request {rid} differs from its single call"
+        else:
+            tdt = torch.float32 if np.dtype(r.dtype) == np.float32 else torch.float64
+            alone, _ = codec.decompress(r.src, tdt, r.eb, numel=r.numel)
+            torch.cuda.synchronize()
+            assert torch.equal(keep[rid].view(torch.uint8), alone.view(torch.uint8)), f"request {rid} differs"
+    stats.update({"rate_rps": args.serve_rate, "sizes": f"log-uniform 64 KiB .. {args.serve_max_mb:g} MiB",
+                  "streams": 4, "checked_alone": len(keep), "identical": True})
+    del pools, reqs, pb, keep
+    torch.cuda.empty_cache()
+    return stats
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -349,6 +450,8 @@ def main():
     torch.cuda.empty_cache()
     if not args.no_batch:
         out["batched"] = bench_batch(args, dev)
+    if not args.no_serve:
+        out["serve"] = bench_serve(args, dev)
     if x_host is not None:
         out["cpu_baseline"] = cpu_baseline(x_host, args.cpu_mb)
     print(json.dumps(out), flush=True)

@@ -73,13 +73,21 @@ class Graph:
 class Batch:
     """A planned batched call over n arrays in one direction."""
 
-    def __init__(self, items, decompress, device, keep=()):
+    def __init__(self, items, decompress, device, keep=(), stream=None):
+        """stream: when given, the descriptors are copied (pinned, asynchronous)
+        and the workspace allocated in that stream's order, for launches on it."""
         self.items = np.ascontiguousarray(items)
         self.n = len(self.items)
         self.decompress = bool(decompress)
         self.tiles, self.ws_bytes = plan(self.items, decompress)
-        self.d_items = torch.from_numpy(self.items.view(np.uint8).copy()).to(device)
-        self.ws = torch.empty(max(self.ws_bytes, 16), dtype=torch.uint8, device=device)
+        raw = torch.from_numpy(self.items.view(np.uint8).copy())
+        if stream is None:
+            self.d_items = raw.to(device)
+            self.ws = torch.empty(max(self.ws_bytes, 16), dtype=torch.uint8, device=device)
+        else:
+            with torch.cuda.stream(stream):
+                self.d_items = raw.pin_memory().to(device, non_blocking=True)
+                self.ws = torch.empty(max(self.ws_bytes, 16), dtype=torch.uint8, device=device)
         self.keep = list(keep)
 
     def launch(self, grid=0, stream=None):

@@ -37,9 +37,11 @@ def parse():
     p.add_argument("--cpu-mb", type=int, default=64, help="CPU baseline sample size in MiB")
     p.add_argument("--seed", type=int, default=1)
     p.add_argument("--encode-variant", type=int, default=0,
-                   help="0 pipelined (default), 1 pipelined at the compiler's register choice, 2 one wave per chunk")
+                   help="0 default, 1 pipelined 256-thread, 2 the same at the compiler's register choice, 3 one "
+                        "wave per chunk, 4 one wave per chunk pipelined over its chunks")
     p.add_argument("--encode-grid", type=int, default=0, help="encoder persistent grid (0 = automatic)")
     p.add_argument("--decode-grid", type=int, default=0, help="decoder grid (0 = one workgroup per chunk)")
+    p.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     p.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     p.add_argument("--batch-arrays", type=int, default=4096)
     p.add_argument("--batch-max-mb", type=float, default=64.0, help="largest array of the batched workload")
@@ -331,11 +333,23 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    x = config_field(1, seed=args.seed, device=dev, scale=args.scale).reshape(-1).contiguous()
+    tuning = None
     if args.encode_variant or args.encode_grid or args.decode_grid:
         from sleek_amd import _lib
 
         _lib.check(_lib.load().sleek_gpu_set_tuning(args.encode_variant, args.encode_grid, args.decode_grid), "tuning")
-    x = config_field(1, seed=args.seed, device=dev, scale=args.scale).reshape(-1).contiguous()
+        tuning = {"encode_variant": args.encode_variant, "encode_grid": args.encode_grid,
+                  "decode_grid": args.decode_grid, "source": "command line"}
+    elif not args.no_autotune:
+        # launch shape for this input, searched before the timed region (bytes are checked invariant)
+        from sleek_amd import autotune
+
+        best, trace = autotune.tune(x, None, cache=False)
+        tuning = dict(best, source="autotune", candidates=len(trace))
+        for cfg, te, td in trace:
+            print(f"autotune {json.dumps(cfg)} encode {te * 1e3:.4f} ms decode {td * 1e3:.4f} ms",
+                  file=sys.stderr, flush=True)
     nbytes = x.numel() * 4
     rng = finite_range(x)
     ebs = [None] + [float(np.float32(np.float32(rng) * np.float32(e))) for e in MODES[1:]]
@@ -440,6 +454,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
                      "alg_bytes_per_launch": alg, "avg_launch_ms": round(tms, 4)},
         "per_mode": per,
+        "tuning": tuning,
         "memcpy_d2d": {"ms": round(mtm * 1e3, 4), "effective_GBps": round(nbytes / mtm / 1e9, 1),
                        "traffic_GBps": round(2 * nbytes / mtm / 1e9, 1),
                        "hbm_frac": round(2 * nbytes / mtm / HBM_PEAK, 4)},

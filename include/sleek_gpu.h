@@ -102,9 +102,11 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
                                long long capacity, long long* d_decsize, void* d_workspace,
                                long long workspace_bytes, int blocks, void* stream);
 /* Launch-shape knobs; no reference counterpart.  encode_variant: 0 =
- * pipelined 256-thread encoder at a 4-waves-per-SIMD register budget (the
- * default), 1 = the same at the compiler's register choice, 2 = one wave
- * per chunk; encode_grid / decode_grid: grid sizes (0 = automatic).  They
+ * default (currently 4), 1 = pipelined 256-thread encoder at a
+ * 4-waves-per-SIMD register budget, 2 = the same at the compiler's register
+ * choice, 3 = one wave per chunk, 4 = one wave per chunk pipelined over the
+ * wave's chunks;
+ * encode_grid / decode_grid: grid sizes (0 = automatic).  They
  * change speed only, never the bytes.  Applies to calls made after it
  * returns; SLEEK_EINVAL for an unknown variant or a negative size. */
 int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid);

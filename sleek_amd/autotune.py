@@ -1,0 +1,202 @@
+"""Autotuner of the codec's launch shape.
+
+The stream format fixes the chunk (16 KiB) and everything the bytes depend
+on; the tuner searches only what changes speed:
+
+* encoder workgroup shape: the pipelined 256-thread encoder (at a 4-wave
+  register budget or the compiler's choice) or one 64-thread wave per chunk,
+  and its persistent grid (a fraction of the resident grid: a larger grid
+  only adds look-back waits, and the library caps it);
+* look-back geometry: chunks per look-back group (8..64) and the window of
+  groups the first group-level poll watches (1..64);
+* decoder grid (one workgroup per chunk, or a multiple of the resident grid).
+
+Coordinate descent: encoder shape x grid at the default look-back, then
+look-back group x window, then the decoder grid.  Every candidate's stream
+and decoded bytes are compared with the default configuration's: a
+configuration that changed a byte would be a bug, and the tuner raises.
+Results are cached in a JSON file keyed by device, dtype, mode and size
+bucket (``SLEEK_TUNE_CACHE``, default ~/.cache/sleek_amd/tune.json) and
+applied with ``apply``.
+"""
+import json
+import math
+import os
+
+DEFAULT = {"encode_variant": 0, "encode_grid_mult": 0.0, "lb_group": 64, "lb_window": 4, "decode_grid_mult": 0.0}
+
+ENCODE_SHAPES = [(0, 0.0), (4, 0.75), (1, 0.0), (1, 0.75), (3, 0.0)]
+MIN_GAIN = 0.01   # a candidate must beat the best so far by 1 % (timing noise is about that)
+LOOKBACK = [(16, 4), (32, 4), (64, 2), (64, 8), (64, 16)]
+DECODE_GRIDS = [1.0, 2.0, 4.0]
+
+
+def cache_path():
+    return os.environ.get("SLEEK_TUNE_CACHE", os.path.join(os.path.expanduser("~"), ".cache", "sleek_amd", "tune.json"))
+
+
+def size_bucket(nbytes):
+    """Power-of-4 size bucket (16 KiB, 64 KiB, ... ) of an input size."""
+    return int(4 ** max(7, math.ceil(math.log(max(nbytes, 1), 4))))
+
+
+def key(device_name, elem_bytes, lossy, nbytes):
+    return f"{device_name}|f{8 * elem_bytes}|{'lossy' if lossy else 'lossless'}|{size_bucket(nbytes)}"
+
+
+def load_cache(path=None):
+    p = path or cache_path()
+    try:
+        with open(p) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return {}
+
+
+def save_cache(cache, path=None):
+    p = path or cache_path()
+    os.makedirs(os.path.dirname(p) or ".", exist_ok=True)
+    tmp = p + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(cache, f, indent=1, sort_keys=True)
+    os.replace(tmp, p)
+
+
+def search(measure, encode_shapes=ENCODE_SHAPES, lookback=LOOKBACK, decode_grids=DECODE_GRIDS):
+    """Coordinate descent over the knobs.  measure(config) -> (encode_s,
+    decode_s, digest); digest identifies the bytes produced.  Returns (best
+    config, trace of (config, encode_s, decode_s))."""
+    trace = []
+    base_enc, base_dec, base_digest = measure(dict(DEFAULT))
+    trace.append((dict(DEFAULT), base_enc, base_dec))
+
+    def run(cfg):
+        e, d, dig = measure(cfg)
+        if dig != base_digest:
+            raise RuntimeError(f"tuning configuration {cfg} changed the output bytes")
+        trace.append((dict(cfg), e, d))
+        return e, d
+
+    best = dict(DEFAULT)
+    best_enc, best_dec = base_enc, base_dec
+    for variant, mult in encode_shapes:
+        cfg = dict(best, encode_variant=variant, encode_grid_mult=mult)
+        if cfg == best:
+            continue
+        e, _ = run(cfg)
+        if e < best_enc * (1 - MIN_GAIN):
+            best, best_enc = cfg, e
+    for group, window in lookback:
+        cfg = dict(best, lb_group=group, lb_window=window)
+        if cfg == best:
+            continue
+        e, _ = run(cfg)
+        if e < best_enc * (1 - MIN_GAIN):
+            best, best_enc = cfg, e
+    for mult in decode_grids:
+        cfg = dict(best, decode_grid_mult=mult)
+        _, d = run(cfg)
+        if d < best_dec * (1 - MIN_GAIN):
+            best, best_dec = cfg, d
+    return best, trace
+
+
+def apply(cfg):
+    """Sets the library's knobs to a configuration.  Grids are multiples of
+    the variant's resident encoder grid (decoder: twice that)."""
+    from . import _lib
+
+    lib = _lib.load()
+    variant = int(cfg["encode_variant"])
+    _lib.check(lib.sleek_gpu_set_tuning(variant, 0, 0), "tuning")
+    res = max(1, int(lib.sleek_gpu_thread_blocks()))
+    eg = int(round(cfg["encode_grid_mult"] * res)) if cfg["encode_grid_mult"] > 0 else 0
+    dg = int(round(cfg["decode_grid_mult"] * res * 2)) if cfg["decode_grid_mult"] > 0 else 0
+    _lib.check(lib.sleek_gpu_set_tuning(variant, eg, dg), "tuning")
+    _lib.check(lib.sleek_gpu_set_lookback(int(cfg["lb_group"]), int(cfg["lb_window"])), "lookback")
+
+
+def tune(x, eb=None, reps=5, cache=True, path=None):
+    """Tunes the single-array codec on the CUDA tensor x (a representative
+    input) and applies the result; returns (config, trace)."""
+    import torch
+
+    from . import codec
+
+    dev = x.device
+    nbytes = x.numel() * x.element_size()
+    k = key(torch.cuda.get_device_name(dev), x.element_size(), eb is not None, nbytes)
+    if cache:
+        c = load_cache(path)
+        if k in c:
+            apply(c[k])
+            return c[k], []
+    out = torch.empty(codec.upper_bound_size(nbytes), dtype=torch.uint8, device=dev)
+    enc = torch.zeros(1, dtype=torch.int64, device=dev)
+    dec = torch.empty_like(x)
+    dsz = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws = codec.Workspace(nbytes, dev)
+    stream = torch.cuda.current_stream()
+
+    ref = {}
+
+    def measure(cfg):
+        apply(cfg)
+        te, td = [], []
+        for _ in range(reps + 1):
+            a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+            a0.record(stream)
+            codec.compress(x, eb, out=out, encsize=enc, workspace=ws)
+            a1.record(stream)
+            codec.decompress(out, x.dtype, eb, out=dec, decsize=dsz, workspace=ws)
+            a2.record(stream)
+            torch.cuda.synchronize()
+            te.append(a0.elapsed_time(a1) / 1e3)
+            td.append(a1.elapsed_time(a2) / 1e3)
+        n = int(enc.item())
+        if not ref:
+            ref["n"], ref["out"], ref["dec"] = n, out[:n].clone(), dec.clone()
+            digest = "reference"
+        else:
+            same = n == ref["n"] and torch.equal(out[:n], ref["out"]) and \
+                torch.equal(dec.view(torch.uint8), ref["dec"].view(torch.uint8))
+            digest = "reference" if same else f"changed (stream length {n})"
+        return float(sorted(te[1:])[reps // 2]), float(sorted(td[1:])[reps // 2]), digest
+
+    try:
+        best, trace = search(measure)
+    except Exception:
+        apply(DEFAULT)
+        raise
+    apply(best)
+    if cache:
+        c = load_cache(path)
+        c[k] = best
+        save_cache(c, path)
+    return best, trace
+
+
+def main():
+    import argparse
+
+    import numpy as np
+    import torch
+
+    from .datagen import array_field
+
+    p = argparse.ArgumentParser(description="tune the SLEEK codec's launch shape for an input size")
+    p.add_argument("--bytes", type=int, default=512 << 20)
+    p.add_argument("--dtype", choices=["f32", "f64"], default="f32")
+    p.add_argument("--eb", type=float, default=None, help="absolute error bound (default lossless)")
+    p.add_argument("--no-cache", action="store_true")
+    a = p.parse_args()
+    dt = np.float32 if a.dtype == "f32" else np.float64
+    x = array_field(a.bytes, dt, seed=1, device=torch.device("cuda"))
+    best, trace = tune(x, a.eb, cache=not a.no_cache)
+    for cfg, e, d in trace:
+        print(json.dumps({"config": cfg, "encode_ms": round(e * 1e3, 4), "decode_ms": round(d * 1e3, 4)}))
+    print(json.dumps({"best": best}))
+
+
+if __name__ == "__main__":
+    main()

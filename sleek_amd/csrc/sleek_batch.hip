@@ -150,7 +150,9 @@ hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long lon
 {
   if (n <= 0) return hipSuccess;
   uint32_t* const map = (uint32_t*)(ws + batch_map_word(n));
-  long long g = grid > 0 ? grid : batch_resident_grid(decompress);
+  const long long res = batch_resident_grid(decompress);
+  long long g = grid > 0 ? grid : res;
+  if (!decompress && res > 0 && g > res) g = res;   // the encoder's look-back: no more than the resident set
   if (g <= 0) g = 1024;
   if (g > tiles) g = tiles;
   hipLaunchKernelGGL(batch_prep_kernel, dim3(n), dim3(256), 0, stream, items, map, ws);

@@ -42,7 +42,7 @@ long long ws_words_for(long long insize)
 long long ws_bytes_for(long long insize) { return ws_words_for(insize) * 8; }
 
 // tuning knobs (sleek_gpu_set_tuning); none changes the bytes produced
-int g_encode_variant = sleek::kEncodePipelined;
+int g_encode_variant = sleek::kEncodeDefault;
 int g_encode_grid = 0;
 int g_decode_grid = 0;
 int g_lb_shift = 6;    // look-back groups of 64 chunks
@@ -83,7 +83,14 @@ int grid_for(int blocks, int elem_bytes, bool lossy, bool decode, long long nchu
 {
   long long g = blocks;
   if (g <= 0) g = decode ? g_decode_grid : g_encode_grid;
-  if (g <= 0) g = decode ? nchunks : sleek::resident_grid(elem_bytes, lossy, false, g_encode_variant);
+  if (decode) {
+    if (g <= 0) g = nchunks;
+  } else {
+    // the encoder's look-back waits on chunks in launch order: a grid beyond
+    // the resident set only adds waits, so it is capped there
+    const int res = sleek::resident_grid(elem_bytes, lossy, false, g_encode_variant);
+    if (g <= 0 || (res > 0 && g > res)) g = res;
+  }
   if (g <= 0) g = 1024;
   if (nchunks >= 0 && g > nchunks) g = nchunks > 0 ? nchunks : 1;
   if (g > (1 << 20)) g = 1 << 20;

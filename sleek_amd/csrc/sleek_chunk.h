@@ -125,9 +125,12 @@ constexpr uint64_t kHelpTicks = 10000;   // 100 us
 // Polls back off exponentially: the words live at device scope, and many
 // waiting waves polling at full rate would compete with the data traffic.
 // Workgroups take chunks in launch order, so the words waited on are
-// normally published within a few round trips; a word still missing after
-// kHelpTicks is computed from the input and published by the waiter, so
-// the look-back finishes even when a predecessor's workgroup is not running.
+// normally published within a few round trips.  After kHelpTicks without
+// progress the waiter computes missing words from the input and publishes
+// them, one after another without further waits, until others' words arrive
+// again; so the look-back finishes even when a predecessor's workgroup is
+// not running (a co-running kernel holds the CUs, or the grid exceeds the
+// resident set).
 template <typename W, bool LOSSY>
 __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long c, long long nchunks)
 {
@@ -139,8 +142,9 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
   uint64_t partial = 0, gacc = 0;
   uint64_t since = __builtin_amdgcn_s_memrealtime();
   int naps = 1;
+  bool helping = false;   // the wait timed out: size missing words from the input, one after another
   while (true) {
-    bool progressed = false;
+    bool progressed = false, helped = false;
     if (!have_partial) {
       const uint64_t f = (lane < q) ? flag_load(s.chunk + (c - 1 - lane)) : kAgg;   // beyond the group: 0
       const uint64_t inc = __ballot((f >> 62) == 2);
@@ -154,14 +158,14 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         have_partial = true;
         progressed = true;
       }
-      if (!progressed && __builtin_amdgcn_s_memrealtime() - since > kHelpTicks) {
+      if (!progressed && (helping || __builtin_amdgcn_s_memrealtime() - since > kHelpTicks)) {
         const uint64_t miss = __ballot(lane < q && (f >> 62) == 0);
         if (miss) {
           const long long jc = c - __ffsll((unsigned long long)miss);
           const uint32_t sz = wave_chunk_size<W, LOSSY>(a, jc);
           if (lane == 0) flag_max(s.chunk + jc, kAgg | sz);
         }
-        progressed = true;
+        helping = helped = true;
       }
     } else {
       const long long gj = gend - 1 - lane;
@@ -178,67 +182,70 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         gwin = min(2 * gwin, 64);
         progressed = true;
       }
-      if (!progressed && __builtin_amdgcn_s_memrealtime() - since > kHelpTicks) {
+      if (!progressed && (helping || __builtin_amdgcn_s_memrealtime() - since > kHelpTicks)) {
         const uint64_t miss = __ballot((fg >> 62) == 0);
         if (miss) {
           const long long gm = gend - __ffsll((unsigned long long)miss);
           const uint64_t sz = wave_group_size<W, LOSSY>(a, s, gm, nchunks);
           if (lane == 0) flag_max(s.group + gm, (gm == 0 ? kInc : kAgg) | sz);
         }
-        progressed = true;
+        helping = helped = true;
       }
     }
-    if (progressed) {
+    if (progressed) {   // others' words arrived: wait for them again rather than help
       since = __builtin_amdgcn_s_memrealtime();
       naps = 1;
+      helping = false;
       continue;
     }
+    if (helped) continue;
     for (int i = 0; i < naps; i++) __builtin_amdgcn_s_sleep(2);
     if (naps < 32) naps *= 2;
   }
 }
 
-// One chunk of the encoder, by one 64-thread workgroup (a single wave).
-// Lane l holds the 8-byte slice l of each of the chunk's 32 subchunks, so
-// every per-subchunk quantity is wave-uniform; the chunk's raw slices stay
-// in registers (64 VGPRs) and its codes are computed twice (bit lengths,
-// then packing), which costs less than holding them as well.  stage is the
-// wave's private LDS buffer (kStageBytes).  Publishes the chunk's size,
-// runs its look-back, stores its payload, its size-table entry and, for
-// chunk 0 and the last chunk, the header and *a.outsize.
-template <typename W, bool LOSSY>
-__device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const ScanState& st, long long c, long long nchunks,
-                                                  uint32_t* stage)
+// ---- one-wave chunk coder ----
+// A single wave codes a chunk.  Lane l holds the 8-byte slice l of each of
+// the chunk's 32 subchunks (64 VGPRs), so every per-subchunk quantity is
+// wave-uniform.  Codes are computed twice (bit lengths, then packing), which
+// costs less than holding them as well.
+
+// The raw slices of chunk c, zero beyond the input.
+__device__ __forceinline__ void load_raw(const EncodeArgs& a, long long c, uint2 raw[kSub])
 {
-  constexpr int PL = Word<W>::PerLane;
-  const int lane = threadIdx.x;
-  const long long data_base = 8 + 2 * nchunks;
-  uint16_t* const sizes = (uint16_t*)(a.out + 8);
-  const W offs = (W)a.offs;
-  const int loff = lane * 8;
-  for (int i = lane; i < kChunk / 16 + 1; i += 64) ((uint4*)stage)[i] = make_uint4(0u, 0u, 0u, 0u);
+  const int lane = threadIdx.x & 63;
   const long long base = c * kChunk;
   const int osize = (int)min((long long)kChunk, a.insize - base);
-
-  // the chunk's raw slices stay in registers (64 VGPRs); codes are
-  // computed twice (bit lengths, then packing), which is cheaper than
-  // holding them next to the raw values
-  uint2 raw[kSub];
+  const uint2* const p = (const uint2*)(a.in + base) + lane;
   if (osize == kChunk) {
     // full chunk: one pointer, constant offsets, no masking
-    const uint2* const p = (const uint2*)(a.in + base) + lane;
 #pragma unroll
     for (int s = 0; s < kSub; s++) raw[s] = p[s * (kSubBytes / 8)];
   } else {
-    const uint2* const p = (const uint2*)(a.in + base) + lane;
-    const int rem = osize - loff;
+    const int rem = osize - lane * 8;
 #pragma unroll
     for (int s = 0; s < kSub; s++) raw[s] = load_lane_rem(p, s, kSubBytes / 8, rem);
   }
+}
 
-  uint32_t L[8];   // bit lengths of the 32 subchunks, byte-packed (wave-uniform)
+// What the stream holds for one chunk: bit lengths (byte-packed, uniform),
+// payload bits, input bytes and stored bytes (csize < osize: packed).
+struct ChunkShape {
+  uint32_t L[8];
+  int total_bits;
+  int osize;
+  int csize;
+};
+
+template <typename W, bool LOSSY>
+__device__ __forceinline__ ChunkShape shape_raw(const EncodeArgs& a, long long c, const uint2 raw[kSub])
+{
+  constexpr int PL = Word<W>::PerLane;
+  const W offs = (W)a.offs;
+  ChunkShape sh;
+  sh.osize = (int)min((long long)kChunk, a.insize - c * kChunk);
 #pragma unroll
-  for (int i = 0; i < 8; i++) L[i] = 0;
+  for (int i = 0; i < 8; i++) sh.L[i] = 0;
 #pragma unroll
   for (int s = 0; s < kSub; s++) {
     W v[PL];
@@ -246,34 +253,55 @@ __device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const Sca
     W orv = 0;
 #pragma unroll
     for (int k = 0; k < PL; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
-    L[s / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (s % 4));
+    sh.L[s / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (s % 4));
   }
   uint32_t nb = 0;
 #pragma unroll
-  for (int i = 0; i < 8; i++) nb += byte_sum(L[i]);
-  const int total_bits = (int)nb * Word<W>::PerSub;
-  const int newsize = kSub + 2 + total_bits / 8;
-  const bool packed = newsize < osize;
-  const int csize = packed ? newsize : osize;
+  for (int i = 0; i < 8; i++) nb += byte_sum(sh.L[i]);
+  sh.total_bits = (int)nb * Word<W>::PerSub;
+  const int newsize = kSub + 2 + sh.total_bits / 8;
+  sh.csize = newsize < sh.osize ? newsize : sh.osize;
+  return sh;
+}
 
-  unsigned long long grp_old = 0;
-  if (lane == 0) {
+// Publishes chunk c's size (lane 0): its look-back word, its size-table
+// entry and its group's counter; the chunk that completes its group
+// publishes the group's aggregate.
+__device__ __forceinline__ void publish_chunk(const EncodeArgs& a, const ScanState& st, long long c, long long nchunks,
+                                              int csize)
+{
+  if ((threadIdx.x & 63) == 0) {
     flag_max(st.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
-    grp_old = __hip_atomic_fetch_add(st.count + (c >> st.gshift), (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-    sizes[c] = (uint16_t)csize;
+    ((uint16_t*)(a.out + 8))[c] = (uint16_t)csize;
+    const unsigned long long old = __hip_atomic_fetch_add(st.count + (c >> st.gshift), (1ull << 32) | (unsigned)csize,
+                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const long long g = c >> st.gshift;
+    if ((long long)(old >> 32) + 1 == group_members(st, g, nchunks))
+      flag_max(st.group + g, (g == 0 ? kInc : kAgg) | ((old & 0xffffffffull) + (unsigned)csize));
   }
-  __syncthreads();   // stage cleared before it is packed into
-#pragma unroll
-  for (int s = 0; s < kSub; s++) reissue_pair(raw[s]);
+}
 
-  if (packed) {
-    if (lane < 8) stage[lane] = L[lane];
-    if (lane == 8) stage[8 + total_bits / 32] = (uint32_t)osize;
+// The chunk's payload into the wave's LDS buffer: the packed form (32
+// bit-length bytes, LSB-first fields, int16 trailer) or the raw bytes.
+template <typename W, bool LOSSY>
+__device__ __forceinline__ void pack_raw(const EncodeArgs& a, const uint2 raw[kSub], const ChunkShape& sh,
+                                         uint32_t* stage)
+{
+  constexpr int PL = Word<W>::PerLane;
+  const int lane = threadIdx.x & 63;
+  const W offs = (W)a.offs;
+  if (sh.csize < sh.osize) {
+    // clear the payload's dwords (rounded to 16 bytes), then OR the fields in
+    for (int i = lane; i < (sh.csize + 15) / 16; i += 64) ((uint4*)stage)[i] = make_uint4(0u, 0u, 0u, 0u);
+    __syncthreads();
+    if (lane == 0) {   // the 32 bit-length bytes (uniform values: no per-lane indexing)
+      ((uint4*)stage)[0] = make_uint4(sh.L[0], sh.L[1], sh.L[2], sh.L[3]);
+      ((uint4*)stage)[1] = make_uint4(sh.L[4], sh.L[5], sh.L[6], sh.L[7]);
+    }
     int p0 = 8 * kSub;
 #pragma unroll
     for (int s = 0; s < kSub; s++) {
-      const int ln = sub_length(L, s);
+      const int ln = sub_length(sh.L, s);
       if (ln) {
         W v[PL];
         lane_values(raw[s], v);
@@ -283,27 +311,47 @@ __device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const Sca
       }
       p0 += ln * Word<W>::PerSub;
     }
+    __syncthreads();
+    if (lane == 0) stage[8 + sh.total_bits / 32] = (uint32_t)sh.osize;   // the trailer's dword follows the fields
   } else {
     // raw chunk: the original bytes
 #pragma unroll
     for (int s = 0; s < kSub; s++) ((uint2*)stage)[lane + s * (kSubBytes / 8)] = raw[s];
   }
+  __syncthreads();
+}
 
-  if (lane == 0) {
-    const long long g = c >> st.gshift;
-    if ((long long)(grp_old >> 32) + 1 == group_members(st, g, nchunks))
-      flag_max(st.group + g, (g == 0 ? kInc : kAgg) | ((grp_old & 0xffffffffull) + (unsigned)csize));
-  }
+// Look-back for chunk c, its inclusive word, and the store of its payload
+// (plus the header for chunk 0 and *outsize for the last chunk).
+template <typename W, bool LOSSY>
+__device__ __forceinline__ void finish_chunk(const EncodeArgs& a, const ScanState& st, long long c, long long nchunks,
+                                             int csize, const uint32_t* stage)
+{
+  const int lane = threadIdx.x & 63;
+  const long long data_base = 8 + 2 * nchunks;
   const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
   if (lane == 0 && c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)csize);
-  __syncthreads();   // packed payload complete
-
   wave_store_unaligned(a.out + data_base + (long long)excl, stage, csize);
   if (lane == 0) {
     if (c == 0) *(long long*)a.out = a.insize;
     if (c == nchunks - 1) *a.outsize = data_base + (long long)excl + csize;
   }
-  __syncthreads();   // stage read by the stores before the next chunk clears it
+  __syncthreads();   // stage read by the stores before it is reused
+}
+
+// One chunk, start to end, by one wave (the batched encoder's tile).
+template <typename W, bool LOSSY>
+__device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const ScanState& st, long long c, long long nchunks,
+                                                  uint32_t* stage)
+{
+  uint2 raw[kSub];
+  load_raw(a, c, raw);
+  const ChunkShape sh = shape_raw<W, LOSSY>(a, c, raw);
+  publish_chunk(a, st, c, nchunks, sh.csize);
+#pragma unroll
+  for (int s = 0; s < kSub; s++) reissue_pair(raw[s]);
+  pack_raw<W, LOSSY>(a, raw, sh, stage);
+  finish_chunk<W, LOSSY>(a, st, c, nchunks, sh.csize, stage);
 }
 
 // Validity of a stream header against the caller's limits (uniform).

@@ -214,6 +214,54 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
   for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) encode_chunk_wave<W, LOSSY>(a, st, c, nchunks, stage);
 }
 
+// Encoder, one wave per chunk, pipelined across a wave's chunks (launch
+// order, stride = grid).  Iteration for chunk c, whose size was published
+// in the previous iteration:
+//   issue the loads of the next chunk n -> pack c into LDS (covers the load
+//   latency) -> size n and publish it -> look-back for c -> store c.
+// So a chunk's size is published a whole iteration before its look-back
+// runs, and its predecessors' sizes, published by the other waves at about
+// the same time, are there when it looks: the look-back rarely waits.  Two
+// chunks' raw slices are live at once (128 VGPRs), two waves per SIMD.
+template <typename W, bool LOSSY>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void encode_wave2_kernel(EncodeArgs a)
+{
+  __shared__ uint32_t stage[kStageBytes / 4];
+  const long long nchunks = (a.insize + kChunk - 1) / kChunk;
+  if (nchunks == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *(long long*)a.out = 0;
+      *a.outsize = 8;
+    }
+    return;
+  }
+  const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
+  long long c = blockIdx.x;
+  if (c >= nchunks) return;
+  uint2 cur[kSub];
+  load_raw(a, c, cur);
+  ChunkShape sc = shape_raw<W, LOSSY>(a, c, cur);
+  publish_chunk(a, st, c, nchunks, sc.csize);
+  while (true) {
+    const long long n = c + gridDim.x;
+    const bool more = n < nchunks;
+    uint2 nxt[kSub];
+    if (more) load_raw(a, n, nxt);
+    pack_raw<W, LOSSY>(a, cur, sc, stage);
+    ChunkShape sn;
+    if (more) {
+      sn = shape_raw<W, LOSSY>(a, n, nxt);
+      publish_chunk(a, st, n, nchunks, sn.csize);
+    }
+    finish_chunk<W, LOSSY>(a, st, c, nchunks, sc.csize, stage);
+    if (!more) break;
+    c = n;
+    sc = sn;
+#pragma unroll
+    for (int s = 0; s < kSub; s++) cur[s] = nxt[s];
+  }
+}
+
 __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
 {
   __shared__ long long wsum[kScanThreads / 64];
@@ -237,7 +285,9 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 template <typename W, bool LOSSY>
 static hipError_t launch_encode(const EncodeArgs& a, int grid, int variant, hipStream_t stream)
 {
+  variant = resolve_variant(variant);
   if (variant == kEncodeWave) hipLaunchKernelGGL((encode_wave_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
+  else if (variant == kEncodeWavePipe) hipLaunchKernelGGL((encode_wave2_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
   else if (variant == kEncodeFree) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 1>), dim3(grid), dim3(kThreads), 0, stream, a);
   else hipLaunchKernelGGL((encode_kernel<W, LOSSY, 4>), dim3(grid), dim3(kThreads), 0, stream, a);
   return hipGetLastError();
@@ -266,7 +316,9 @@ hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int gr
 template <typename W, bool LOSSY>
 static const void* encode_fn(int variant)
 {
+  variant = resolve_variant(variant);
   if (variant == kEncodeWave) return (const void*)encode_wave_kernel<W, LOSSY>;
+  if (variant == kEncodeWavePipe) return (const void*)encode_wave2_kernel<W, LOSSY>;
   return variant == kEncodeFree ? (const void*)encode_kernel<W, LOSSY, 1> : (const void*)encode_kernel<W, LOSSY, 4>;
 }
 
@@ -283,7 +335,8 @@ int resident_grid(int elem_bytes, bool lossy, bool decode, int variant)
     fn = elem_bytes == 4 ? (lossy ? encode_fn<uint32_t, true>(variant) : encode_fn<uint32_t, false>(variant))
                          : (lossy ? encode_fn<uint64_t, true>(variant) : encode_fn<uint64_t, false>(variant));
   }
-  const int threads = (!decode && variant == kEncodeWave) ? 64 : kThreads;
+  const int rv = resolve_variant(variant);
+  const int threads = (!decode && (rv == kEncodeWave || rv == kEncodeWavePipe)) ? 64 : kThreads;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);
 }

@@ -68,5 +68,5 @@ def test_barriers_in_single_outer_loop(asm):
     bdec = [k for k in b if "batch_decode_kernel" in k]
     assert len(bdec) == 1 and b[bdec[0]] == [1] * 8, b.get(bdec[0] if bdec else None)
     # single-wave workgroups (one-wave encoder, batched encoder) need no barrier instruction
-    assert not [k for k in b if "wave_kernel" in k or "batch_encode" in k]
+    assert not [k for k in b if "wave" in k or "batch_encode" in k]
     assert all(d == 1 for v in b.values() for d in v), b

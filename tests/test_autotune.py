@@ -9,7 +9,7 @@ def fake_measure(costs, digests=None):
 
     def measure(cfg):
         calls.append(dict(cfg))
-        e = 1.0 - 0.1 * (cfg["encode_variant"] == 2) - 0.05 * (cfg["lb_window"] == 8)
+        e = 1.0 - 0.1 * (cfg["encode_variant"] == 3) - 0.05 * (cfg["lb_window"] == 8)
         d = 1.0 - 0.2 * (cfg["decode_grid_mult"] == 2.0)
         dig = "same" if digests is None else digests(cfg)
         return e * costs, d * costs, dig
@@ -19,7 +19,7 @@ def fake_measure(costs, digests=None):
 def test_search_picks_fastest_per_coordinate():
     measure, calls = fake_measure(1e-3)
     best, trace = at.search(measure)
-    assert best["encode_variant"] == 2 and best["lb_window"] == 8 and best["decode_grid_mult"] == 2.0
+    assert best["encode_variant"] == 3 and best["lb_window"] == 8 and best["decode_grid_mult"] == 2.0
     assert calls[0] == at.DEFAULT and len(trace) == len(calls)
 
 

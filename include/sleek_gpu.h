@@ -146,7 +146,7 @@ typedef struct sleek_gpu_batch_item {
   int lossy;                /* 0 or 1 */
   /* filled by sleek_gpu_batch_plan */
   long long tile0;          /* first tile of this array */
-  long long tiles;          /* its tiles (>= 1) */
+  long long tiles;          /* its tiles (16 KiB chunks; 0 for an empty array) */
   long long ws_word;        /* first 8-byte word of its workspace block */
   long long ws_words;       /* words in that block */
 } sleek_gpu_batch_item;     /* 96 bytes */

@@ -1,9 +1,9 @@
 // Batched SLEEK kernels: many arrays (mixed float32/float64, lossless/lossy)
 // in one launch per direction.
 //
-// A tile is one 16 KiB chunk of one array.  The host plan numbers every
-// array's tiles consecutively (tile0, tiles) and gives it a workspace block
-// with the single-array layout.  batch_prep_kernel clears the blocks and
+// A tile is one 16 KiB chunk of one array (an empty array has none).  The
+// host plan numbers every array's tiles consecutively (tile0, tiles) and
+// gives it a workspace block with the single-array layout.  batch_prep_kernel clears the blocks and
 // writes the tile map (tile -> array), so a launch needs no host work
 // proportional to the tile count and records no memset.  The coders then
 // walk the tiles in launch order (stride = grid), read the array's
@@ -16,15 +16,21 @@
 
 namespace sleek {
 
-// One workgroup per array: clear its workspace block (16-byte stores) and
-// write its tiles into the map.
-__global__ __launch_bounds__(256) void batch_prep_kernel(const BatchItem* items, uint32_t* tile_map, uint64_t* ws)
+// One workgroup per array: clear its workspace block (16-byte stores),
+// write its tiles into the map, and give an empty array to be compressed
+// its 8-byte zero header (it has no tile).
+__global__ __launch_bounds__(256) void batch_prep_kernel(const BatchItem* items, uint32_t* tile_map, uint64_t* ws,
+                                                         int compress)
 {
   const BatchItem* it = items + blockIdx.x;
   const long long w0 = it->ws_word, half = it->ws_words / 2, t0 = it->tile0, nt = it->tiles;
   uint4* const z = (uint4*)(ws + w0);
   for (long long k = threadIdx.x; k < half; k += 256) z[k] = make_uint4(0u, 0u, 0u, 0u);
   for (long long k = threadIdx.x; k < nt; k += 256) tile_map[t0 + k] = blockIdx.x;
+  if (compress && it->src_bytes == 0 && threadIdx.x == 0) {
+    *(long long*)it->dst = 0;
+    *it->d_size = 8;
+  }
 }
 
 __device__ __forceinline__ int item_kind(const BatchItem* it)
@@ -32,40 +38,98 @@ __device__ __forceinline__ int item_kind(const BatchItem* it)
   return (it->elem_bytes == 8 ? 2 : 0) + (it->lossy ? 1 : 0);
 }
 
-// Encoder: one wave per tile (encode_chunk_wave).
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void batch_encode_kernel(
-    const BatchItem* items, const uint32_t* tile_map, long long tiles, uint64_t* ws, int lb_shift, int lb_window)
+// A tile of the batched encoder: its array's arguments and scan state.
+struct EncTile {
+  EncodeArgs a;
+  ScanState st;
+  long long c, nchunks;
+  int kind;
+};
+
+__device__ __forceinline__ EncTile enc_tile(const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map,
+                                            long long t, uint64_t* ws, int lb_shift, int lb_window)
+{
+  const BatchItem* it = items + uni(tile_map[t]);
+  EncTile e;
+  e.a.in = it->src;
+  e.a.insize = it->src_bytes;
+  e.a.out = it->dst;
+  e.a.outsize = it->d_size;
+  e.a.ws = ws + it->ws_word;
+  e.a.eb_e = it->eb_e;
+  e.a.thr_e = it->thr_e;
+  e.a.offs = it->offs;
+  e.a.lb_shift = lb_shift;
+  e.a.lb_window = lb_window;
+  e.nchunks = (e.a.insize + kChunk - 1) / kChunk;
+  e.c = t - it->tile0;
+  e.kind = item_kind(it);
+  e.st = scan_state(e.a.ws, e.nchunks, lb_shift);
+  return e;
+}
+
+__device__ __forceinline__ ChunkShape enc_shape(const EncTile& e, uint2 slot[kSub])
+{
+  if (e.kind == 0) return shape_codes<uint32_t, false>(e.a, e.c, slot);
+  if (e.kind == 1) return shape_codes<uint32_t, true>(e.a, e.c, slot);
+  if (e.kind == 2) return shape_codes<uint64_t, false>(e.a, e.c, slot);
+  return shape_codes<uint64_t, true>(e.a, e.c, slot);
+}
+
+__device__ __forceinline__ void enc_pack(const EncTile& e, const uint2 slot[kSub], const ChunkShape& sh, uint32_t* stage)
+{
+  if (e.kind == 0) pack_codes<uint32_t, false>(e.a, e.c, slot, sh, stage);
+  else if (e.kind == 1) pack_codes<uint32_t, true>(e.a, e.c, slot, sh, stage);
+  else if (e.kind == 2) pack_codes<uint64_t, false>(e.a, e.c, slot, sh, stage);
+  else pack_codes<uint64_t, true>(e.a, e.c, slot, sh, stage);
+}
+
+__device__ __forceinline__ void enc_finish(const EncTile& e, int csize, const uint32_t* stage)
+{
+  // the element type only matters to a look-back that sizes a missing chunk itself
+  if (e.kind == 0) finish_chunk<uint32_t, false>(e.a, e.st, e.c, e.nchunks, csize, stage);
+  else if (e.kind == 1) finish_chunk<uint32_t, true>(e.a, e.st, e.c, e.nchunks, csize, stage);
+  else if (e.kind == 2) finish_chunk<uint64_t, false>(e.a, e.st, e.c, e.nchunks, csize, stage);
+  else finish_chunk<uint64_t, true>(e.a, e.st, e.c, e.nchunks, csize, stage);
+}
+
+// Encoder: one wave per tile, pipelined over the wave's tiles like the
+// single-array encoder: the next tile's slices load while the current one
+// is packed, its codes and size are made before the current look-back.
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void batch_encode_kernel(
+    const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map, long long tiles, uint64_t* ws,
+    int lb_shift, int lb_window)
 {
   __shared__ uint32_t stage[kStageBytes / 4];
-  for (long long t = blockIdx.x; t < tiles; t += gridDim.x) {
-    const BatchItem* it = items + uni(tile_map[t]);
-    EncodeArgs a;
-    a.in = it->src;
-    a.insize = it->src_bytes;
-    a.out = it->dst;
-    a.outsize = it->d_size;
-    a.ws = ws + it->ws_word;
-    a.eb_e = it->eb_e;
-    a.thr_e = it->thr_e;
-    a.offs = it->offs;
-    a.lb_shift = lb_shift;
-    a.lb_window = lb_window;
-    const long long nchunks = (a.insize + kChunk - 1) / kChunk;
-    const long long c = t - it->tile0;
-    const int kind = item_kind(it);
-    if (nchunks == 0) {
-      // an empty array: the 8-byte zero header
-      if (threadIdx.x == 0) {
-        *(long long*)a.out = 0;
-        *a.outsize = 8;
-      }
-    } else {
-      const ScanState st = scan_state(a.ws, nchunks, lb_shift);
-      if (kind == 0) encode_chunk_wave<uint32_t, false>(a, st, c, nchunks, stage);
-      else if (kind == 1) encode_chunk_wave<uint32_t, true>(a, st, c, nchunks, stage);
-      else if (kind == 2) encode_chunk_wave<uint64_t, false>(a, st, c, nchunks, stage);
-      else encode_chunk_wave<uint64_t, true>(a, st, c, nchunks, stage);
+  long long t = blockIdx.x;
+  if (t >= tiles) return;
+  EncTile ec = enc_tile(items, tile_map, t, ws, lb_shift, lb_window);
+  uint2 cur[kSub];
+  load_raw(ec.a, ec.c, cur);
+  ChunkShape sc = enc_shape(ec, cur);
+  publish_chunk(ec.a, ec.st, ec.c, ec.nchunks, sc.csize);
+  while (true) {
+    const long long n = t + gridDim.x;
+    const bool more = n < tiles;
+    EncTile en;
+    uint2 nxt[kSub];
+    if (more) {
+      en = enc_tile(items, tile_map, n, ws, lb_shift, lb_window);
+      load_raw(en.a, en.c, nxt);
     }
+    enc_pack(ec, cur, sc, stage);
+    ChunkShape sn;
+    if (more) {
+      sn = enc_shape(en, nxt);
+      publish_chunk(en.a, en.st, en.c, en.nchunks, sn.csize);
+    }
+    enc_finish(ec, sc.csize, stage);
+    if (!more) break;
+    t = n;
+    ec = en;
+    sc = sn;
+#pragma unroll
+    for (int s = 0; s < kSub; s++) cur[s] = nxt[s];
   }
 }
 
@@ -154,12 +218,13 @@ hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long lon
   long long g = grid > 0 ? grid : res;
   if (!decompress && res > 0 && g > res) g = res;   // the encoder's look-back: no more than the resident set
   if (g <= 0) g = 1024;
-  if (g > tiles) g = tiles;
-  hipLaunchKernelGGL(batch_prep_kernel, dim3(n), dim3(256), 0, stream, items, map, ws);
+  if (g > tiles) g = tiles > 0 ? tiles : 1;
+  hipLaunchKernelGGL(batch_prep_kernel, dim3(n), dim3(256), 0, stream, items, map, ws, decompress ? 0 : 1);
   if (decompress) {
     hipLaunchKernelGGL(batch_offsets_kernel, dim3(n), dim3(kScanThreads), 0, stream, items, ws);
-    hipLaunchKernelGGL(batch_decode_kernel, dim3((unsigned)g), dim3(kThreads), 0, stream, items, map, tiles, ws);
-  } else {
+    if (tiles > 0)
+      hipLaunchKernelGGL(batch_decode_kernel, dim3((unsigned)g), dim3(kThreads), 0, stream, items, map, tiles, ws);
+  } else if (tiles > 0) {
     hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)g), dim3(64), 0, stream, items, map, tiles, ws, lb_shift,
                        lb_window);
   }

@@ -408,7 +408,7 @@ int sleek_gpu_batch_plan(sleek_gpu_batch_item* items, int n, int decompress, lon
     }
     const long long ch = chunks_of(bytes);
     it.tile0 = tile;
-    it.tiles = ch > 0 ? ch : 1;
+    it.tiles = ch;
     it.ws_words = ws_words_for(bytes);
     tile += it.tiles;
     if (tile > 0xffffffffll) return SLEEK_EINVAL;
@@ -427,7 +427,7 @@ int sleek_gpu_batch_plan(sleek_gpu_batch_item* items, int n, int decompress, lon
 static int batch_async(bool decompress, const sleek_gpu_batch_item* d_items, int n, long long tiles, void* ws,
                        long long ws_bytes, int grid, void* stream)
 {
-  if (n < 0 || tiles < 0 || (n > 0 && (!d_items || !ws || tiles < n))) return SLEEK_EINVAL;
+  if (n < 0 || tiles < 0 || (n > 0 && (!d_items || !ws))) return SLEEK_EINVAL;
   if (n == 0) return SLEEK_OK;
   if (!aligned(ws, 16) || !aligned(d_items, 8)) return SLEEK_EALIGN;
   if (ws_bytes < (sleek::batch_map_word(n) + (tiles + 1) / 2) * 8) return SLEEK_EWORKSPACE;

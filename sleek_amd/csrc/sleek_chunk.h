@@ -207,8 +207,8 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
 // ---- one-wave chunk coder ----
 // A single wave codes a chunk.  Lane l holds the 8-byte slice l of each of
 // the chunk's 32 subchunks (64 VGPRs), so every per-subchunk quantity is
-// wave-uniform.  Codes are computed twice (bit lengths, then packing), which
-// costs less than holding them as well.
+// wave-uniform.  The slices are turned into codes in place once; packing
+// reads the codes.
 
 // The raw slices of chunk c, zero beyond the input.
 __device__ __forceinline__ void load_raw(const EncodeArgs& a, long long c, uint2 raw[kSub])
@@ -237,8 +237,9 @@ struct ChunkShape {
   int csize;
 };
 
+// Codes of the chunk's values, in place, and the chunk's shape.
 template <typename W, bool LOSSY>
-__device__ __forceinline__ ChunkShape shape_raw(const EncodeArgs& a, long long c, const uint2 raw[kSub])
+__device__ __forceinline__ ChunkShape shape_codes(const EncodeArgs& a, long long c, uint2 slot[kSub])
 {
   constexpr int PL = Word<W>::PerLane;
   const W offs = (W)a.offs;
@@ -249,10 +250,14 @@ __device__ __forceinline__ ChunkShape shape_raw(const EncodeArgs& a, long long c
 #pragma unroll
   for (int s = 0; s < kSub; s++) {
     W v[PL];
-    lane_values(raw[s], v);
+    lane_values(slot[s], v);
     W orv = 0;
 #pragma unroll
-    for (int k = 0; k < PL; k++) orv |= to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
+    for (int k = 0; k < PL; k++) {
+      v[k] = to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
+      orv |= v[k];
+    }
+    slot[s] = values_lane(v);
     sh.L[s / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (s % 4));
   }
   uint32_t nb = 0;
@@ -282,14 +287,15 @@ __device__ __forceinline__ void publish_chunk(const EncodeArgs& a, const ScanSta
 }
 
 // The chunk's payload into the wave's LDS buffer: the packed form (32
-// bit-length bytes, LSB-first fields, int16 trailer) or the raw bytes.
+// bit-length bytes, LSB-first fields, int16 trailer) from the codes, or
+// the raw bytes: lossless codes invert exactly, lossy ones do not, so a
+// raw lossy chunk is read again from the input.
 template <typename W, bool LOSSY>
-__device__ __forceinline__ void pack_raw(const EncodeArgs& a, const uint2 raw[kSub], const ChunkShape& sh,
-                                         uint32_t* stage)
+__device__ __forceinline__ void pack_codes(const EncodeArgs& a, long long c, const uint2 code[kSub],
+                                           const ChunkShape& sh, uint32_t* stage)
 {
   constexpr int PL = Word<W>::PerLane;
   const int lane = threadIdx.x & 63;
-  const W offs = (W)a.offs;
   if (sh.csize < sh.osize) {
     // clear the payload's dwords (rounded to 16 bytes), then OR the fields in
     for (int i = lane; i < (sh.csize + 15) / 16; i += 64) ((uint4*)stage)[i] = make_uint4(0u, 0u, 0u, 0u);
@@ -304,19 +310,28 @@ __device__ __forceinline__ void pack_raw(const EncodeArgs& a, const uint2 raw[kS
       const int ln = sub_length(sh.L, s);
       if (ln) {
         W v[PL];
-        lane_values(raw[s], v);
-#pragma unroll
-        for (int k = 0; k < PL; k++) v[k] = to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
+        lane_values(code[s], v);
         or_field(stage, p0 + lane * PL * ln, lane_field(v, ln));
       }
       p0 += ln * Word<W>::PerSub;
     }
     __syncthreads();
     if (lane == 0) stage[8 + sh.total_bits / 32] = (uint32_t)sh.osize;   // the trailer's dword follows the fields
+  } else if constexpr (LOSSY) {
+    // rare: streamed through a few registers at a time
+    const uint2* const p = (const uint2*)(a.in + c * kChunk) + lane;
+    const int rem = sh.osize - lane * 8;
+#pragma unroll 4
+    for (int s = 0; s < kSub; s++) ((uint2*)stage)[lane + s * (kSubBytes / 8)] = load_lane_rem(p, s, kSubBytes / 8, rem);
   } else {
-    // raw chunk: the original bytes
 #pragma unroll
-    for (int s = 0; s < kSub; s++) ((uint2*)stage)[lane + s * (kSubBytes / 8)] = raw[s];
+    for (int s = 0; s < kSub; s++) {
+      W v[PL];
+      lane_values(code[s], v);
+#pragma unroll
+      for (int k = 0; k < PL; k++) v[k] = lossless_value<W>(v[k]);
+      ((uint2*)stage)[lane + s * (kSubBytes / 8)] = values_lane(v);
+    }
   }
   __syncthreads();
 }
@@ -344,13 +359,11 @@ template <typename W, bool LOSSY>
 __device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const ScanState& st, long long c, long long nchunks,
                                                   uint32_t* stage)
 {
-  uint2 raw[kSub];
-  load_raw(a, c, raw);
-  const ChunkShape sh = shape_raw<W, LOSSY>(a, c, raw);
+  uint2 slot[kSub];
+  load_raw(a, c, slot);
+  const ChunkShape sh = shape_codes<W, LOSSY>(a, c, slot);
   publish_chunk(a, st, c, nchunks, sh.csize);
-#pragma unroll
-  for (int s = 0; s < kSub; s++) reissue_pair(raw[s]);
-  pack_raw<W, LOSSY>(a, raw, sh, stage);
+  pack_codes<W, LOSSY>(a, c, slot, sh, stage);
   finish_chunk<W, LOSSY>(a, st, c, nchunks, sh.csize, stage);
 }
 

@@ -64,36 +64,39 @@ __device__ __forceinline__ int clz_w(uint64_t x) { return __clzll(x); }
 
 // Lossless code (reference sleek_common_cpu.h:152-169): rotate the sign into
 // bit 0, re-centre a non-zero exponent around 1.0, zigzag-fold.
+//
+// Closed form used here: with v = rotl1(x) and u = v - kOne (kOne = the
+// lowest exponent bit after the rotation), the reference's case analysis
+// on the exponent e (0 / 1..2^(E-1) / above) reduces to
+//   code = (u << 1) + (t << (M + 2))  XOR  (t - 1),   t = top bit of u,
+// since 1 <= e <= 2^(E-1) exactly when u's top bit is clear.  It equals
+// the reference transform on all 2^32 float32 and all float64 exponent
+// classes (checked against the oracle).
 template <typename W>
-__device__ __forceinline__ W lossless_code(W v)
+__host__ __device__ __forceinline__ W lossless_code(W x)
 {
   constexpr int TB = Word<W>::TB;
-  constexpr W kTop = (W)((((W)1 << Word<W>::EXPB) - 1)) << (Word<W>::MANB + 1);
-  constexpr W kHalf = (W)1 << (TB - 1);
   constexpr W kOne = (W)1 << (Word<W>::MANB + 1);
-  v = (v << 1) | (v >> (TB - 1));
-  if (v & kTop) {
-    v -= kHalf;
-    if (!(v & kTop) || (v & kHalf)) v -= kOne;
-  }
-  const W fold = (W)0 - (v >> (TB - 1));
-  return (v << 1) ^ fold;
+  const W u = ((x << 1) | (x >> (TB - 1))) - kOne;
+  const W t = u >> (TB - 1);
+  return ((u << 1) + (t << (Word<W>::MANB + 2))) ^ (t - 1);
 }
 
-// Inverse of lossless_code (reference sleek_common_cpu.h:317-335).
+// Inverse of lossless_code (reference sleek_common_cpu.h:317-335): bit 0
+// of the code is the fold; an odd code undoes (u << 1) ^ ~0, an even one
+// (u << 1) + 2 kOne, whose rotated value v gets its top bit back exactly
+// when its exponent field is non-zero (c1 >= kOne).
 template <typename W>
-__device__ __forceinline__ W lossless_value(W v)
+__host__ __device__ __forceinline__ W lossless_value(W c)
 {
   constexpr int TB = Word<W>::TB;
-  constexpr W kTop = (W)((((W)1 << Word<W>::EXPB) - 1)) << (Word<W>::MANB + 1);
   constexpr W kHalf = (W)1 << (TB - 1);
   constexpr W kOne = (W)1 << (Word<W>::MANB + 1);
-  v = (v >> 1) ^ ((W)0 - (v & 1));
-  if (v & kTop) {
-    if (v & kHalf) v += kOne;
-    v += kHalf;
-  }
-  return (v << (TB - 1)) | (v >> 1);
+  const W m = (W)0 - (c & 1);
+  const W c1 = c >> 1;
+  const W sel = (c1 + (kHalf - kOne)) & kHalf;
+  const W v = (c1 ^ m) + ((m & (kHalf + kOne)) | (~m & sel));
+  return (v >> 1) | (v << (TB - 1));
 }
 
 // Error-bounded code (reference sleek_common_cpu.h:393-419): magnitudes at or
@@ -185,12 +188,17 @@ __device__ __forceinline__ uint32_t wave_or(uint32_t v)
 // Bit length of the largest code over a wave (the bit length of the OR of
 // the codes), wave-uniform.
 __device__ __forceinline__ int wave_bit_length(uint32_t orv) { const uint32_t o = wave_or(orv); return o ? 32 - __clz(o) : 0; }
+// 64-bit codes: each lane's own bit length (no branch), then the wave's max.
 __device__ __forceinline__ int wave_bit_length(uint64_t orv)
 {
-  const uint32_t hi = wave_or((uint32_t)(orv >> 32));
-  if (hi) return 64 - __clz(hi);
-  const uint32_t lo = wave_or((uint32_t)orv);
-  return lo ? 32 - __clz(lo) : 0;
+  const uint32_t hi = (uint32_t)(orv >> 32), lo = (uint32_t)orv;
+  uint32_t b = hi ? 64u - (uint32_t)__clz(hi) : (lo ? 32u - (uint32_t)__clz(lo) : 0u);
+  b = max(b, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)b, 0xB1, 0xF, 0xF, false));   // quad_perm [1,0,3,2]
+  b = max(b, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)b, 0x4E, 0xF, 0xF, false));   // quad_perm [2,3,0,1]
+  b = max(b, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)b, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  b = max(b, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)b, 0x140, 0xF, 0xF, false));  // row_mirror
+  return (int)max(max((uint32_t)__builtin_amdgcn_readlane((int)b, 0), (uint32_t)__builtin_amdgcn_readlane((int)b, 16)),
+                  max((uint32_t)__builtin_amdgcn_readlane((int)b, 32), (uint32_t)__builtin_amdgcn_readlane((int)b, 48)));
 }
 
 __device__ __forceinline__ int bit_length(uint32_t x) { return 32 - __clz(x); }

@@ -196,10 +196,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 
 // Encoder, one wave per chunk: a workgroup is a single wave that codes its
 // chunks (launch order, stride = grid) with encode_chunk_wave.  The private
-// 16 KiB LDS buffer puts nine such waves on a CU; the look-back and the
-// load latencies of one wave are covered by the others' arithmetic.
+// 16 KiB LDS buffer allows nine such waves on a CU, registers eight; the
+// look-back and the load latencies of one wave are covered by the others'
+// arithmetic.
 template <typename W, bool LOSSY>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void encode_wave_kernel(EncodeArgs a)
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void encode_wave_kernel(EncodeArgs a)
 {
   __shared__ uint32_t stage[kStageBytes / 4];
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
@@ -215,10 +216,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 3))) void
 }
 
 // Encoder, one wave per chunk, pipelined across a wave's chunks (launch
-// order, stride = grid).  Iteration for chunk c, whose size was published
-// in the previous iteration:
+// order, stride = grid).  Iteration for chunk c, whose codes and size were
+// made in the previous iteration:
 //   issue the loads of the next chunk n -> pack c into LDS (covers the load
-//   latency) -> size n and publish it -> look-back for c -> store c.
+//   latency) -> code n, size it and publish its size -> look-back for c ->
+//   store c.
 // So a chunk's size is published a whole iteration before its look-back
 // runs, and its predecessors' sizes, published by the other waves at about
 // the same time, are there when it looks: the look-back rarely waits.  Two
@@ -240,17 +242,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   if (c >= nchunks) return;
   uint2 cur[kSub];
   load_raw(a, c, cur);
-  ChunkShape sc = shape_raw<W, LOSSY>(a, c, cur);
+  ChunkShape sc = shape_codes<W, LOSSY>(a, c, cur);
   publish_chunk(a, st, c, nchunks, sc.csize);
   while (true) {
     const long long n = c + gridDim.x;
     const bool more = n < nchunks;
     uint2 nxt[kSub];
     if (more) load_raw(a, n, nxt);
-    pack_raw<W, LOSSY>(a, cur, sc, stage);
+    pack_codes<W, LOSSY>(a, c, cur, sc, stage);
     ChunkShape sn;
     if (more) {
-      sn = shape_raw<W, LOSSY>(a, n, nxt);
+      sn = shape_codes<W, LOSSY>(a, n, nxt);
       publish_chunk(a, st, n, nchunks, sn.csize);
     }
     finish_chunk<W, LOSSY>(a, st, c, nchunks, sc.csize, stage);

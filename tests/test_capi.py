@@ -137,13 +137,13 @@ def test_batch_plan_layout(decompress):
     tiles, wsb = plan(items, decompress)
     even = lambda w: (w + 1) & ~1
     t = 0
-    word = 16 + even(len(specs)) + even((sum(max(1, -(-b // 16384)) for b, _, _ in specs) + 1) // 2)
+    word = 16 + even(len(specs)) + even((sum(-(-b // 16384) for b, _, _ in specs) + 1) // 2)
     for it, (b, _, _) in zip(items, specs):
         ch = -(-b // 16384)
-        assert it["tile0"] == t and it["tiles"] == max(1, ch)
+        assert it["tile0"] == t and it["tiles"] == ch
         assert it["ws_words"] == even(16 + ch + 2 * (-(-ch // 8)))   # groups as small as 8 chunks
         assert it["ws_word"] == word and it["ws_word"] % 2 == 0
-        t += max(1, ch)
+        t += ch
         word += it["ws_words"]
     assert tiles == t and wsb == word * 8
 

@@ -153,6 +153,28 @@ def test_lookback_geometry_invariance(gpu, variant):
         lib.sleek_gpu_set_tuning(0, 0, 0)
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_every_exponent_class(gpu, dt):
+    """Every (sign, exponent) pair with random mantissas, lossless and at several
+    bounds: the transforms' case analysis runs only on the exponent."""
+    say(f"every exponent class {np.dtype(dt).name}")
+    rng = np.random.default_rng(8)
+    if dt == np.float32:
+        se = np.arange(512, dtype=np.uint32)
+        bits = (se[:, None] << np.uint32(23)) | rng.integers(0, 1 << 23, (512, 8), dtype=np.uint32)
+        a = bits.reshape(-1).view(np.float32)
+    else:
+        se = np.arange(4096, dtype=np.uint64)
+        bits = (se[:, None] << np.uint64(52)) | rng.integers(0, 1 << 52, (4096, 4), dtype=np.uint64)
+        a = bits.reshape(-1).view(np.float64)
+    for eb in (None, 1e-30, 1e-3, 1e3):
+        ref = po.compress(a, eb)
+        s, d = gpu_roundtrip(a, eb)
+        assert s.tobytes() == ref.tobytes(), eb
+        od, rc = po.decompress(ref, dt, eb)
+        assert rc == 0 and d.tobytes() == od.tobytes(), eb
+
+
 @pytest.mark.parametrize("config", [1, 2])
 def test_config_fields_scaled(gpu, config):
     """The benchmark recipes (sinusoids, ramp, lognormal noise, specials) at 1/4 scale."""

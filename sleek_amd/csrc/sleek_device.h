@@ -57,8 +57,8 @@ template <> struct Word<uint64_t> {
   static constexpr int PerSub = 64;
 };
 
-__device__ __forceinline__ int clz_w(uint32_t x) { return __clz(x); }
-__device__ __forceinline__ int clz_w(uint64_t x) { return __clzll(x); }
+__host__ __device__ __forceinline__ int clz_w(uint32_t x) { return x ? __builtin_clz(x) : 32; }
+__host__ __device__ __forceinline__ int clz_w(uint64_t x) { return x ? __builtin_clzll(x) : 64; }
 
 // ---- value transforms.  The steps are fixed by the stream format ----
 
@@ -103,28 +103,33 @@ __host__ __device__ __forceinline__ W lossless_value(W c)
 // above 2^thr keep their bits (minus an offset), magnitudes in [2^eb, 2^thr)
 // are rounded to bins of width 2^(eb+1), smaller ones become 0; the sign is
 // folded into bit 0 with both zeros mapping to 0.  Unsigned arithmetic equals
-// the reference's wrapping signed arithmetic modulo 2^TB.  Both arms are
-// computed and selected (shift amounts clamped to the word), so a wave never
-// diverges on the value's magnitude.
+// the reference's wrapping signed arithmetic modulo 2^TB.  Branch-free:
+//  * below 2^eb the bin formula itself gives 0 (the shift, clamped to TB-1,
+//    exceeds the mantissa's width), so no third case is selected;
+//  * (q << 1 | !sign) - (q != 0) equals 2q - min(q, sign).
+// Equal to the reference on all 2^32 float32 patterns for every class of
+// bound (checked against the oracle).
 template <typename W>
-__device__ __forceinline__ W lossy_code(W val, int eb_e, int thr_e, W offs)
+__host__ __device__ __forceinline__ W lossy_code(W val, int eb_e, int thr_e, W offs)
 {
+  (void)eb_e;   // implied by thr_e
   constexpr int TB = Word<W>::TB;
   constexpr int EM = Word<W>::EXPB + Word<W>::MANB;
   constexpr int M = Word<W>::MANB;
   const W mag = val & (((W)1 << EM) - 1);
   const int e = (int)(mag >> M);
-  const int sh = min(max(thr_e - e, 1), TB - 1);          // in [1, M+1] where it is used
+  const int d = thr_e - 1 - e;                       // shift - 1
+  const int sh1 = d < 0 ? 0 : (d > TB - 2 ? TB - 2 : d);
   const W mant = (val & (((W)1 << M) - 1)) | ((W)1 << M);
-  const W binned = (mant + ((W)1 << (sh - 1))) >> sh;
-  W enc = (e >= thr_e) ? (W)(mag - offs) : ((e >= eb_e) ? binned : (W)0);
-  enc = (enc << 1) | ((~val) >> EM);
-  return enc - (enc != 0 ? (W)1 : (W)0);
+  const W binned = ((mant >> sh1) + 1) >> 1;         // (mant + 2^(sh-1)) >> sh
+  const W q = (e >= thr_e) ? (W)(mag - offs) : binned;
+  const W sign = val >> (TB - 1);
+  return (q << 1) - (q < sign ? q : sign);
 }
 
 // Inverse of lossy_code (reference sleek_common_cpu.h:559-591), branch-free.
 template <typename W>
-__device__ __forceinline__ W lossy_value(W enc, int thr_e, W offs)
+__host__ __device__ __forceinline__ W lossy_value(W enc, int thr_e, W offs)
 {
   constexpr int TB = Word<W>::TB;
   constexpr int M = Word<W>::MANB;

@@ -37,27 +37,37 @@ __device__ __forceinline__ void store_slice(uint8_t* dst, int off, int osize, ui
 }
 
 // The 8-byte slice at byte off of a chunk of osize bytes (osize is a
-// multiple of 4), zero beyond osize; aligned, so it never crosses a page.
+// multiple of 4), zero beyond osize.  Branch-free: an invalid slice is
+// loaded from the chunk's first slice (always inside the input) and
+// zeroed, so the loads of a partial chunk are all in flight at once
+// instead of one conditional load after another.
 __device__ __forceinline__ uint2 load_lane(const uint8_t* src, int off, int osize)
 {
-  if (off >= osize) return make_uint2(0u, 0u);
-  uint2 q = *(const uint2*)(src + off);
-  if (osize - off < 8) q.y = 0u;
+  const bool ok = off < osize;
+  uint2 q = *(const uint2*)(src + (ok ? off : 0));
+  // masks the compiler cannot see through keep the load unconditional
+  // (a known zero/all-ones mask would let it sink the load into a branch)
+  uint32_t mx = 0u - (uint32_t)ok, my = 0u - (uint32_t)(osize - off >= 8);
+  asm volatile("" : "+v"(mx), "+v"(my));
+  q.x &= mx;
+  q.y &= my;
   return q;
 }
 
 // Slice s (stride sub_stride 8-byte words) of a lane whose chunk has rem
 // valid bytes from the lane's first slice on; zero where invalid.  Written
 // with one lane pointer and constant offsets so that no per-slice offset is
-// kept in registers across the chunk loop.
-__device__ __forceinline__ uint2 load_lane_rem(const uint2* lp, int s, int sub_stride, int rem)
+// kept in registers across the chunk loop; an invalid slice is loaded from
+// safe (the chunk's first slice) and zeroed, without a branch.
+__device__ __forceinline__ uint2 load_lane_rem(const uint2* lp, const uint2* safe, int s, int sub_stride, int rem)
 {
   const int b = s * sub_stride * 8;
-  uint2 q = make_uint2(0u, 0u);
-  if (rem > b) {
-    q = lp[s * sub_stride];
-    if (rem - b < 8) q.y = 0u;
-  }
+  const bool ok = rem > b;
+  uint2 q = *(ok ? lp + s * sub_stride : safe);
+  uint32_t mx = 0u - (uint32_t)ok, my = 0u - (uint32_t)(rem - b >= 8);
+  asm volatile("" : "+v"(mx), "+v"(my));   // opaque masks: see load_lane
+  q.x &= mx;
+  q.y &= my;
   return q;
 }
 
@@ -210,7 +220,10 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
 // wave-uniform.  The slices are turned into codes in place once; packing
 // reads the codes.
 
-// The raw slices of chunk c, zero beyond the input.
+// Issues the loads of chunk c's raw slices (64 VGPRs).  A slice beyond the
+// input (last chunk only) is loaded from the chunk's first slice instead,
+// so every load is unconditional and all 32 are in flight at once; such
+// slices hold garbage until mask_tail zeroes them.
 __device__ __forceinline__ void load_raw(const EncodeArgs& a, long long c, uint2 raw[kSub])
 {
   const int lane = threadIdx.x & 63;
@@ -222,9 +235,29 @@ __device__ __forceinline__ void load_raw(const EncodeArgs& a, long long c, uint2
 #pragma unroll
     for (int s = 0; s < kSub; s++) raw[s] = p[s * (kSubBytes / 8)];
   } else {
-    const int rem = osize - lane * 8;
+    // an opaque copy of the lane's valid byte count chooses the addresses:
+    // the compiler cannot tie it to the masking, so it cannot turn the
+    // loads into conditional ones
+    int rem = osize - lane * 8;
+    asm volatile("" : "+v"(rem));
+    const uint2* const safe = (const uint2*)(a.in + base);
 #pragma unroll
-    for (int s = 0; s < kSub; s++) raw[s] = load_lane_rem(p, s, kSubBytes / 8, rem);
+    for (int s = 0; s < kSub; s++) raw[s] = *(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
+  }
+}
+
+// Zeroes the bytes of chunk c's slices that lie beyond the input.
+__device__ __forceinline__ void mask_tail(const EncodeArgs& a, long long c, uint2 raw[kSub])
+{
+  const int osize = (int)min((long long)kChunk, a.insize - c * kChunk);
+  if (osize < kChunk) {
+    const int rem = osize - (int)(threadIdx.x & 63) * 8;
+#pragma unroll
+    for (int s = 0; s < kSub; s++) {
+      const int r = rem - s * kSubBytes;
+      raw[s].x = r > 0 ? raw[s].x : 0u;
+      raw[s].y = r >= 8 ? raw[s].y : 0u;
+    }
   }
 }
 
@@ -237,7 +270,8 @@ struct ChunkShape {
   int csize;
 };
 
-// Codes of the chunk's values, in place, and the chunk's shape.
+// Codes of the chunk's values, in place (slices as load_raw left them),
+// and the chunk's shape.
 template <typename W, bool LOSSY>
 __device__ __forceinline__ ChunkShape shape_codes(const EncodeArgs& a, long long c, uint2 slot[kSub])
 {
@@ -245,6 +279,7 @@ __device__ __forceinline__ ChunkShape shape_codes(const EncodeArgs& a, long long
   const W offs = (W)a.offs;
   ChunkShape sh;
   sh.osize = (int)min((long long)kChunk, a.insize - c * kChunk);
+  mask_tail(a, c, slot);
 #pragma unroll
   for (int i = 0; i < 8; i++) sh.L[i] = 0;
 #pragma unroll
@@ -319,10 +354,12 @@ __device__ __forceinline__ void pack_codes(const EncodeArgs& a, long long c, con
     if (lane == 0) stage[8 + sh.total_bits / 32] = (uint32_t)sh.osize;   // the trailer's dword follows the fields
   } else if constexpr (LOSSY) {
     // rare: streamed through a few registers at a time
-    const uint2* const p = (const uint2*)(a.in + c * kChunk) + lane;
+    const uint2* const safe = (const uint2*)(a.in + c * kChunk);
+    const uint2* const p = safe + lane;
     const int rem = sh.osize - lane * 8;
 #pragma unroll 4
-    for (int s = 0; s < kSub; s++) ((uint2*)stage)[lane + s * (kSubBytes / 8)] = load_lane_rem(p, s, kSubBytes / 8, rem);
+    for (int s = 0; s < kSub; s++)
+      ((uint2*)stage)[lane + s * (kSubBytes / 8)] = load_lane_rem(p, safe, s, kSubBytes / 8, rem);
   } else {
 #pragma unroll
     for (int s = 0; s < kSub; s++) {

@@ -42,6 +42,7 @@ def parse():
     p.add_argument("--encode-grid", type=int, default=0, help="encoder persistent grid (0 = automatic)")
     p.add_argument("--decode-grid", type=int, default=0, help="decoder grid (0 = one workgroup per chunk)")
     p.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
+    p.add_argument("--no-f64", action="store_true", help="skip the float64 workload")
     p.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     p.add_argument("--batch-arrays", type=int, default=4096)
     p.add_argument("--batch-max-mb", type=float, default=64.0, help="largest array of the batched workload")
@@ -320,6 +321,128 @@ def bench_serve(args, dev):
     return stats
 
 
+def measured_traffic(kernel, shape_scale):
+    """HBM bytes per dispatch of the dominant kernel, as rocprofv3 measured them
+    for this workload (FETCH_SIZE / WRITE_SIZE, calibrated; profiles/), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic_512cubed_f32.json")
+    if shape_scale != 1.0 or not kernel.startswith("encode (lossless)") or not os.path.exists(p):
+        return None, None
+    t = json.load(open(p))
+    return t["fetch_bytes"] + t["write_bytes"], {"fetch_bytes": t["fetch_bytes"], "write_bytes": t["write_bytes"],
+                                                  "kernel": t["kernel"], "source": "profiles/traffic_512cubed_f32.json"}
+
+
+def run_modes(x, ebs, labels, steps, warmup):
+    """Compress then decompress x once per error bound per step, with HIP
+    events on the launch stream around every call.  Returns the per-mode
+    medians, the stream sizes, the timed region's GPU time (s) and its wall
+    time (s).  The lossless round trip is checked exact."""
+    import torch
+
+    from sleek_amd import codec
+
+    dev = x.device
+    nbytes = x.numel() * x.element_size()
+    ub = codec.upper_bound_size(nbytes)
+    stream = torch.cuda.current_stream()
+    bufs = [torch.empty(ub, dtype=torch.uint8, device=dev) for _ in ebs]
+    encs = [torch.zeros(1, dtype=torch.int64, device=dev) for _ in ebs]
+    decs = [torch.empty_like(x) for _ in ebs]
+    dsz = torch.zeros(1, dtype=torch.int64, device=dev)
+    ws = codec.Workspace(nbytes, dev)
+
+    def step(events=None):
+        for i, eb in enumerate(ebs):
+            if events is not None:
+                events[i][0].record(stream)
+            codec.compress(x, eb, out=bufs[i], encsize=encs[i], workspace=ws)
+            if events is not None:
+                events[i][1].record(stream)
+            codec.decompress(bufs[i], x.dtype, eb, out=decs[i], decsize=dsz, workspace=ws)
+            if events is not None:
+                events[i][2].record(stream)
+
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    iv = torch.int32 if x.element_size() == 4 else torch.int64
+    assert torch.equal(decs[0].view(iv), x.view(iv)), "lossless round trip mismatch"
+    assert ws.status() == 0
+    enc_sizes = [int(e.item()) for e in encs]
+
+    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in ebs] for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    start = torch.cuda.Event(enable_timing=True)
+    end = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for k in range(steps):
+        step(ev[k])
+    end.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    t_total = start.elapsed_time(end) / 1e3
+
+    per = []
+    for i, eb in enumerate(ebs):
+        tc = np.median([ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(steps)]) / 1e3
+        td = np.median([ev[k][i][1].elapsed_time(ev[k][i][2]) for k in range(steps)]) / 1e3
+        alg = nbytes + enc_sizes[i]
+        per.append({
+            "mode": labels[i], "eb_abs": eb, "ratio": round(nbytes / enc_sizes[i], 4), "encsize": enc_sizes[i],
+            "compress_ms": round(tc * 1e3, 4), "decompress_ms": round(td * 1e3, 4),
+            "compress_GBps": round(nbytes / tc / 1e9, 1), "decompress_GBps": round(nbytes / td / 1e9, 1),
+            "compress_alg_GBps": round(alg / tc / 1e9, 1), "decompress_alg_GBps": round(alg / td / 1e9, 1),
+            "compress_hbm_frac": round(alg / tc / HBM_PEAK, 4), "decompress_hbm_frac": round(alg / td / HBM_PEAK, 4),
+        })
+    del bufs, decs
+    return per, enc_sizes, t_total, wall
+
+
+def memcpy_d2d(x, reps=10):
+    """hipMemcpy device-to-device of x's bytes (torch copy_): warmed up, median of reps."""
+    import torch
+
+    stream = torch.cuda.current_stream()
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    mt = []
+    for _ in range(reps):
+        a0 = torch.cuda.Event(enable_timing=True)
+        a1 = torch.cuda.Event(enable_timing=True)
+        a0.record(stream)
+        y.copy_(x)
+        a1.record(stream)
+        torch.cuda.synchronize()
+        mt.append(a0.elapsed_time(a1) / 1e3)
+    nbytes = x.numel() * x.element_size()
+    mtm = float(np.median(mt))
+    return {"bytes": nbytes, "ms": round(mtm * 1e3, 4), "effective_GBps": round(nbytes / mtm / 1e9, 1),
+            "traffic_GBps": round(2 * nbytes / mtm / 1e9, 1), "hbm_frac": round(2 * nbytes / mtm / HBM_PEAK, 4)}
+
+
+def bench_f64(args, dev):
+    """The float64 configuration: a 384^3 field (12 sinusoids + lognormal
+    noise, 0.01 % specials), lossless and eb = {1e-6, 1e-9} x range, next to
+    a device-to-device copy of the same bytes."""
+    from sleek_amd.datagen import config_field, finite_range
+
+    x = config_field(2, seed=args.seed, device=dev, scale=args.scale).reshape(-1).contiguous()
+    r = finite_range(x)
+    rels = (1e-6, 1e-9)
+    ebs = [None] + [float(np.float64(r) * np.float64(e)) for e in rels]
+    labels = ["lossless"] + [f"eb={e:g}*range" for e in rels]
+    per, _, t_total, _ = run_modes(x, ebs, labels, max(3, args.steps // 2), 1)
+    nbytes = x.numel() * 8
+    out = {"workload": "384^3 float64, 12 sinusoids + lognormal(-10, 2) noise, 0.01% specials",
+           "bytes": nbytes, "per_mode": per, "memcpy_d2d": memcpy_d2d(x),
+           "GBps": round(2 * nbytes * len(ebs) / (t_total / max(3, args.steps // 2)) / 1e9, 2)}
+    del x
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -353,60 +476,10 @@ def main():
     nbytes = x.numel() * 4
     rng = finite_range(x)
     ebs = [None] + [float(np.float32(np.float32(rng) * np.float32(e))) for e in MODES[1:]]
-    ub = codec.upper_bound_size(nbytes)
-    stream = torch.cuda.current_stream()
-    bufs = [torch.empty(ub, dtype=torch.uint8, device=dev) for _ in ebs]
-    encs = [torch.zeros(1, dtype=torch.int64, device=dev) for _ in ebs]
-    decs = [torch.empty_like(x) for _ in ebs]
-    dsz = torch.zeros(1, dtype=torch.int64, device=dev)
-    ws = codec.Workspace(nbytes, dev)
-
-    def step(events=None):
-        for i, eb in enumerate(ebs):
-            if events is not None:
-                events[i][0].record(stream)
-            codec.compress(x, eb, out=bufs[i], encsize=encs[i], workspace=ws)
-            if events is not None:
-                events[i][1].record(stream)
-            codec.decompress(bufs[i], torch.float32, eb, out=decs[i], decsize=dsz, workspace=ws)
-            if events is not None:
-                events[i][2].record(stream)
-
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    # correctness guard of the timed configuration: lossless round trip is exact
-    assert torch.equal(decs[0].view(torch.int32), x.view(torch.int32)), "lossless round trip mismatch"
-    assert ws.status() == 0
-    enc_sizes = [int(e.item()) for e in encs]
-
-    ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in ebs] for _ in range(args.steps)]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    start = torch.cuda.Event(enable_timing=True)
-    end = torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for k in range(args.steps):
-        step(ev[k])
-    end.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    t_total = start.elapsed_time(end) / 1e3
+    labels = ["lossless"] + [f"eb={m:g}*range" for m in MODES[1:]]
+    per, enc_sizes, t_total, wall = run_modes(x, ebs, labels, args.steps, args.warmup)
     ms_step = t_total / args.steps * 1e3
-
-    per = []
-    for i, eb in enumerate(ebs):
-        tc = np.median([ev[k][i][0].elapsed_time(ev[k][i][1]) for k in range(args.steps)]) / 1e3
-        td = np.median([ev[k][i][1].elapsed_time(ev[k][i][2]) for k in range(args.steps)]) / 1e3
-        alg = nbytes + enc_sizes[i]
-        per.append({
-            "mode": "lossless" if eb is None else f"eb={MODES[i]:g}*range",
-            "eb_abs": eb, "ratio": round(nbytes / enc_sizes[i], 4), "encsize": enc_sizes[i],
-            "compress_ms": round(tc * 1e3, 4), "decompress_ms": round(td * 1e3, 4),
-            "compress_GBps": round(nbytes / tc / 1e9, 1), "decompress_GBps": round(nbytes / td / 1e9, 1),
-            "compress_alg_GBps": round(alg / tc / 1e9, 1), "decompress_alg_GBps": round(alg / td / 1e9, 1),
-            "compress_hbm_frac": round(alg / tc / HBM_PEAK, 4), "decompress_hbm_frac": round(alg / td / HBM_PEAK, 4),
-        })
+    stream = torch.cuda.current_stream()
 
     # dominant kernel: the launch with the largest median time
     cands = []
@@ -416,22 +489,10 @@ def main():
         cands.append((p["decompress_ms"], f"decode ({p['mode']})", alg))
     tms, kname, alg = max(cands)
     achieved = alg / (tms / 1e3) / 1e9
+    traffic, traffic_detail = measured_traffic(kname, args.scale)
 
     # hipMemcpy device-to-device comparator of the same bytes
-    y = torch.empty_like(x)
-    for _ in range(3):
-        y.copy_(x)
-    torch.cuda.synchronize()
-    mt = []
-    for _ in range(10):
-        a0 = torch.cuda.Event(enable_timing=True)
-        a1 = torch.cuda.Event(enable_timing=True)
-        a0.record(stream)
-        y.copy_(x)
-        a1.record(stream)
-        torch.cuda.synchronize()
-        mt.append(a0.elapsed_time(a1) / 1e3)
-    mtm = float(np.median(mt))
+    mc = memcpy_d2d(x)
 
     total_in = 2 * nbytes * len(ebs)
     out = {
@@ -451,18 +512,18 @@ def main():
                                "decompress in lossless and eb={1e-2,1e-3,1e-4}*range modes",
                    "shape": [int(round(512 * args.scale))] * 3, "bytes": nbytes},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None,
-                     "alg_bytes_per_launch": alg, "avg_launch_ms": round(tms, 4)},
+                     "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": traffic,
+                     "traffic_detail": traffic_detail, "alg_bytes_per_launch": alg, "avg_launch_ms": round(tms, 4)},
         "per_mode": per,
         "tuning": tuning,
-        "memcpy_d2d": {"ms": round(mtm * 1e3, 4), "effective_GBps": round(nbytes / mtm / 1e9, 1),
-                       "traffic_GBps": round(2 * nbytes / mtm / 1e9, 1),
-                       "hbm_frac": round(2 * nbytes / mtm / HBM_PEAK, 4)},
+        "memcpy_d2d": mc,
         "wall_s": round(wall, 3),
     }
     x_host = None if args.no_cpu else x.cpu().numpy()
-    del x, bufs, decs, y
+    del x
     torch.cuda.empty_cache()
+    if not args.no_f64:
+        out["float64"] = bench_f64(args, dev)
     if not args.no_batch:
         out["batched"] = bench_batch(args, dev)
     if not args.no_serve:

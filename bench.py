@@ -41,6 +41,7 @@ def parse():
                         "wave per chunk, 4 one wave per chunk pipelined over its chunks")
     p.add_argument("--encode-grid", type=int, default=0, help="encoder persistent grid (0 = automatic)")
     p.add_argument("--decode-grid", type=int, default=0, help="decoder grid (0 = one workgroup per chunk)")
+    p.add_argument("--decoder", type=int, default=-1, help="decoder variant (-1 = library default)")
     p.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     p.add_argument("--no-f64", action="store_true", help="skip the float64 workload")
     p.add_argument("--no-batch", action="store_true", help="skip the batched workload")
@@ -457,6 +458,10 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     x = config_field(1, seed=args.seed, device=dev, scale=args.scale).reshape(-1).contiguous()
+    if args.decoder >= 0:
+        from sleek_amd import _lib
+
+        _lib.check(_lib.load().sleek_gpu_set_decoder(args.decoder), "decoder")
     tuning = None
     if args.encode_variant or args.encode_grid or args.decode_grid:
         from sleek_amd import _lib

@@ -116,6 +116,11 @@ int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid);
  * 64); window_groups = groups watched by the first group-level poll, 1..64
  * (0 = default, 4).  Speed only, never the bytes; applies to later calls. */
 int sleek_gpu_set_lookback(int group_chunks, int window_groups);
+/* Decoder shape (no reference counterpart): 0 = an offset-scan launch, then
+ * one workgroup per chunk (the default); 1 = one persistent launch that
+ * derives each chunk's stream offset from the size table itself.  Same
+ * bytes; applies to later calls. */
+int sleek_gpu_set_decoder(int variant);
 
 /* Synchronises `stream`, then returns SLEEK_OK, SLEEK_ECORRUPT or
  * SLEEK_ECAPACITY from the workspace's status word. */

@@ -9,10 +9,11 @@ on; the tuner searches only what changes speed:
   only adds look-back waits, and the library caps it);
 * look-back geometry: chunks per look-back group (8..64) and the window of
   groups the first group-level poll watches (1..64);
-* decoder grid (one workgroup per chunk, or a multiple of the resident grid).
+* decoder: offset scan + one workgroup per chunk, or the persistent decoder
+  that derives offsets from the size table, and its grid.
 
 Coordinate descent: encoder shape x grid at the default look-back, then
-look-back group x window, then the decoder grid.  Every candidate's stream
+look-back group x window, then the decoder shape.  Every candidate's stream
 and decoded bytes are compared with the default configuration's: a
 configuration that changed a byte would be a bug, and the tuner raises.
 Results are cached in a JSON file keyed by device, dtype, mode and size
@@ -23,12 +24,13 @@ import json
 import math
 import os
 
-DEFAULT = {"encode_variant": 0, "encode_grid_mult": 0.0, "lb_group": 64, "lb_window": 4, "decode_grid_mult": 0.0}
+DEFAULT = {"encode_variant": 0, "encode_grid_mult": 0.0, "lb_group": 64, "lb_window": 4, "decoder": 0,
+           "decode_grid_mult": 0.0}
 
 ENCODE_SHAPES = [(0, 0.0), (4, 0.75), (1, 0.0), (1, 0.75), (3, 0.0)]
 MIN_GAIN = 0.01   # a candidate must beat the best so far by 1 % (timing noise is about that)
 LOOKBACK = [(16, 4), (32, 4), (64, 2), (64, 8), (64, 16)]
-DECODE_GRIDS = [1.0, 2.0, 4.0]
+DECODE_SHAPES = [(0, 1.0), (0, 2.0), (1, 0.0)]   # (decoder, grid multiple; 0.0 = automatic)
 
 
 def cache_path():
@@ -62,7 +64,7 @@ def save_cache(cache, path=None):
     os.replace(tmp, p)
 
 
-def search(measure, encode_shapes=ENCODE_SHAPES, lookback=LOOKBACK, decode_grids=DECODE_GRIDS):
+def search(measure, encode_shapes=ENCODE_SHAPES, lookback=LOOKBACK, decode_shapes=DECODE_SHAPES):
     """Coordinate descent over the knobs.  measure(config) -> (encode_s,
     decode_s, digest); digest identifies the bytes produced.  Returns (best
     config, trace of (config, encode_s, decode_s))."""
@@ -93,8 +95,8 @@ def search(measure, encode_shapes=ENCODE_SHAPES, lookback=LOOKBACK, decode_grids
         e, _ = run(cfg)
         if e < best_enc * (1 - MIN_GAIN):
             best, best_enc = cfg, e
-    for mult in decode_grids:
-        cfg = dict(best, decode_grid_mult=mult)
+    for dec, mult in decode_shapes:
+        cfg = dict(best, decoder=dec, decode_grid_mult=mult)
         _, d = run(cfg)
         if d < best_dec * (1 - MIN_GAIN):
             best, best_dec = cfg, d
@@ -114,6 +116,7 @@ def apply(cfg):
     dg = int(round(cfg["decode_grid_mult"] * res * 2)) if cfg["decode_grid_mult"] > 0 else 0
     _lib.check(lib.sleek_gpu_set_tuning(variant, eg, dg), "tuning")
     _lib.check(lib.sleek_gpu_set_lookback(int(cfg["lb_group"]), int(cfg["lb_window"])), "lookback")
+    _lib.check(lib.sleek_gpu_set_decoder(int(cfg.get("decoder", 0))), "decoder")
 
 
 def tune(x, eb=None, reps=5, cache=True, path=None):

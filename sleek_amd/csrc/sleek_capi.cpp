@@ -45,6 +45,7 @@ long long ws_bytes_for(long long insize) { return ws_words_for(insize) * 8; }
 int g_encode_variant = sleek::kEncodeDefault;
 int g_encode_grid = 0;
 int g_decode_grid = 0;
+int g_decode_variant = sleek::kDecodeScan;
 int g_lb_shift = 6;    // look-back groups of 64 chunks
 int g_lb_window = 4;   // initial look-back window, in groups
 
@@ -84,7 +85,7 @@ int grid_for(int blocks, int elem_bytes, bool lossy, bool decode, long long nchu
   long long g = blocks;
   if (g <= 0) g = decode ? g_decode_grid : g_encode_grid;
   if (decode) {
-    if (g <= 0) g = nchunks;
+    if (g <= 0) g = g_decode_variant == sleek::kDecodeScan ? nchunks : sleek::resident_grid(elem_bytes, lossy, true, 0);
   } else {
     // the encoder's look-back waits on chunks in launch order: a grid beyond
     // the resident set only adds waits, so it is capped there
@@ -155,7 +156,7 @@ int decompress_impl(int elem_bytes, bool lossy, int thr_e, unsigned long long of
   a.thr_e = thr_e;
   a.offs = offs;
   const int grid = grid_for(blocks, elem_bytes, lossy, true, chunks_of(cap));
-  e = sleek::decode_launch(elem_bytes, lossy, a, grid, stream);
+  e = sleek::decode_launch(elem_bytes, lossy, a, grid, g_decode_variant, stream);
   if (e != hipSuccess) return hip_fail(e, "decode launch");
   return SLEEK_OK;
 }
@@ -330,6 +331,14 @@ int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid)
   g_encode_variant = encode_variant;
   g_encode_grid = encode_grid;
   g_decode_grid = decode_grid;
+  return SLEEK_OK;
+}
+
+int sleek_gpu_set_decoder(int variant)
+{
+  if (variant < 0 || variant >= sleek::kDecodeVariants) return SLEEK_EINVAL;
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_decode_variant = variant;
   return SLEEK_OK;
 }
 

@@ -499,8 +499,8 @@ __device__ __forceinline__ void scan_offsets(const DecodeArgs& a, long long* wsu
 // inverse transform -> coalesced stores.  hdr / nchunks come from a header
 // that passed header_ok.  Damage found is reported in the status word.
 template <typename W, bool LOSSY>
-__device__ __forceinline__ void decode_chunk(const DecodeArgs& a, long long hdr, long long nchunks, long long c,
-                                             uint32_t* stage)
+__device__ __forceinline__ void decode_chunk_at(const DecodeArgs& a, long long hdr, long long nchunks, long long c,
+                                                long long excl, uint32_t* stage)
 {
   constexpr int PL = Word<W>::PerLane;
   constexpr int TB = Word<W>::TB;
@@ -509,12 +509,10 @@ __device__ __forceinline__ void decode_chunk(const DecodeArgs& a, long long hdr,
   const int wv = t >> 6;
   const long long data_base = 8 + 2 * nchunks;
   const uint16_t* const sizes = (const uint16_t*)(a.in + 8);
-  const long long* const offs_tab = (const long long*)(a.ws + kWsFlags);
   const W offs = (W)a.offs;
   const long long base = c * kChunk;
   const int osize = (int)min((long long)kChunk, hdr - base);
   const int csize = uni((int)sizes[c]);
-  const long long excl = uni(offs_tab[c]);
   const bool bad = csize > osize || (csize < osize && csize < kSub + 2) ||
                    (a.inlen >= 0 && data_base + excl + csize > a.inlen);   // payload beyond the stream
 
@@ -577,6 +575,48 @@ __device__ __forceinline__ void decode_chunk(const DecodeArgs& a, long long hdr,
     raise_status(a.ws, kStCorrupt);
   }
   __syncthreads();
+}
+
+// The same, with the chunk's offset from the offsets table (offset scan).
+template <typename W, bool LOSSY>
+__device__ __forceinline__ void decode_chunk(const DecodeArgs& a, long long hdr, long long nchunks, long long c,
+                                             uint32_t* stage)
+{
+  const long long* const offs_tab = (const long long*)(a.ws + kWsFlags);
+  decode_chunk_at<W, LOSSY>(a, hdr, nchunks, c, uni(offs_tab[c]), stage);
+}
+
+// Sum of the size-table entries [from, to), to - from <= kWinMax, by one
+// workgroup: table_load issues the loads (entry from + t + kThreads*i for
+// thread t, coalesced), table_sum reduces them once they are needed, so the
+// loads' latency can hide behind other work.
+constexpr int kWinLoads = 8;
+constexpr int kWinMax = kThreads * kWinLoads;
+
+// Entries beyond the window are loaded from entry 0 (always inside the
+// table) and dropped in table_sum, so the loads are unconditional.
+__device__ __forceinline__ void table_load(const uint16_t* sizes, long long from, long long to, uint32_t v[kWinLoads])
+{
+  const int t = threadIdx.x;
+  long long lim = to - from - t;   // window entries at or after this thread's first
+  asm volatile("" : "+v"(lim));    // opaque: keeps the loads unconditional
+#pragma unroll
+  for (int i = 0; i < kWinLoads; i++) v[i] = sizes[lim > (long long)kThreads * i ? from + t + (long long)kThreads * i : 0];
+}
+
+__device__ __forceinline__ long long table_sum(const uint32_t v[kWinLoads], long long from, long long to, long long* red)
+{
+  const long long lim = to - from - (long long)threadIdx.x;
+  uint32_t s = 0;
+#pragma unroll
+  for (int i = 0; i < kWinLoads; i++) s += lim > (long long)kThreads * i ? v[i] : 0u;
+  const uint64_t w = wave_sum((uint64_t)s);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = (long long)w;
+  __syncthreads();
+  long long tot = 0;
+#pragma unroll
+  for (int i = 0; i < kThreads / 64; i++) tot += red[i];
+  return uni(tot);
 }
 
 }  // namespace sleek

@@ -52,7 +52,14 @@ hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int gr
 // Clears `words` 8-byte words with vector stores (no memset node when the
 // caller captures a graph).  p is 8-byte aligned.
 hipError_t zero_launch(uint64_t* p, long long words, hipStream_t stream);
-hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream);
+// Decoder variants: kDecodeScan (default): an offset-scan launch, then one
+// workgroup per chunk; kDecodeFused: one persistent launch that derives
+// chunk offsets from windows of the size table (faster for lossless
+// streams, slower for small lossy payloads).  Same bytes.
+constexpr int kDecodeScan = 0;
+constexpr int kDecodeFused = 1;
+constexpr int kDecodeVariants = 2;
+hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, int variant, hipStream_t stream);
 int resident_grid(int elem_bytes, bool lossy, bool decode, int variant);
 
 // One array of a batched call; the layout of sleek_gpu_batch_item.

@@ -270,6 +270,46 @@ __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
   scan_offsets(a, wsum);
 }
 
+// Decoder without a separate offset scan: workgroup b decodes chunks b,
+// b+G, b+2G, ... (G = grid <= kWinMax) and derives each chunk's stream
+// offset from the previous one it decoded:
+//   excl(c + G) = excl(c) + sizes[c] + ... + sizes[c + G - 1],
+// a window of G table entries (L2-resident) whose loads are issued before
+// chunk c is decoded and summed after.  The first chunk's offset is the sum
+// of the entries before it.  No cross-workgroup dependency, no offsets
+// table, one launch.
+template <typename W, bool LOSSY>
+__global__ __launch_bounds__(kThreads) void decode_fused_kernel(DecodeArgs a)
+{
+  __shared__ uint32_t stage[kStageBytes / 4];
+  __shared__ long long red[2][kThreads / 64];
+  const long long hdr = uni(*(const long long*)a.in);
+  long long nchunks = 0;
+  uint64_t why = 0;
+  const bool ok = header_ok(a, hdr, &nchunks, &why);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (ok) *a.decsize = hdr;
+    else raise_status(a.ws, why);
+  }
+  long long c = blockIdx.x;
+  if (!ok || c >= nchunks) return;
+  const uint16_t* const sizes = (const uint16_t*)(a.in + 8);
+  uint32_t v[kWinLoads];
+  table_load(sizes, 0, c, v);
+  long long excl = table_sum(v, 0, c, red[0]);
+  int par = 1;
+  while (true) {
+    const long long n = c + gridDim.x;
+    const long long wend = min(n, nchunks);
+    table_load(sizes, c, wend, v);
+    decode_chunk_at<W, LOSSY>(a, hdr, nchunks, c, excl, stage);
+    if (n >= nchunks) break;
+    excl += table_sum(v, c, wend, red[par]);
+    par ^= 1;
+    c = n;
+  }
+}
+
 // Decoder: workgroups stride over the chunks in any order.
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
@@ -296,10 +336,15 @@ static hipError_t launch_encode(const EncodeArgs& a, int grid, int variant, hipS
 }
 
 template <typename W, bool LOSSY>
-static hipError_t launch_decode(const DecodeArgs& a, int grid, hipStream_t stream)
+static hipError_t launch_decode(const DecodeArgs& a, int grid, int variant, hipStream_t stream)
 {
-  hipLaunchKernelGGL(offsets_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
-  hipLaunchKernelGGL((decode_kernel<W, LOSSY>), dim3(grid), dim3(kThreads), 0, stream, a);
+  if (variant == kDecodeScan) {
+    hipLaunchKernelGGL(offsets_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
+    hipLaunchKernelGGL((decode_kernel<W, LOSSY>), dim3(grid), dim3(kThreads), 0, stream, a);
+  } else {
+    hipLaunchKernelGGL((decode_fused_kernel<W, LOSSY>), dim3(grid < kWinMax ? grid : kWinMax), dim3(kThreads), 0,
+                       stream, a);
+  }
   return hipGetLastError();
 }
 
@@ -309,10 +354,11 @@ hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int gr
   return lossy ? launch_encode<uint64_t, true>(a, grid, variant, stream) : launch_encode<uint64_t, false>(a, grid, variant, stream);
 }
 
-hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, hipStream_t stream)
+hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, int variant, hipStream_t stream)
 {
-  if (elem_bytes == 4) return lossy ? launch_decode<uint32_t, true>(a, grid, stream) : launch_decode<uint32_t, false>(a, grid, stream);
-  return lossy ? launch_decode<uint64_t, true>(a, grid, stream) : launch_decode<uint64_t, false>(a, grid, stream);
+  if (elem_bytes == 4)
+    return lossy ? launch_decode<uint32_t, true>(a, grid, variant, stream) : launch_decode<uint32_t, false>(a, grid, variant, stream);
+  return lossy ? launch_decode<uint64_t, true>(a, grid, variant, stream) : launch_decode<uint64_t, false>(a, grid, variant, stream);
 }
 
 template <typename W, bool LOSSY>
@@ -331,8 +377,9 @@ int resident_grid(int elem_bytes, bool lossy, bool decode, int variant)
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   const void* fn;
   if (decode) {
-    fn = elem_bytes == 4 ? (lossy ? (const void*)decode_kernel<uint32_t, true> : (const void*)decode_kernel<uint32_t, false>)
-                         : (lossy ? (const void*)decode_kernel<uint64_t, true> : (const void*)decode_kernel<uint64_t, false>);
+    // the fused decoder's occupancy (the offset-scan decoder is launched one workgroup per chunk)
+    fn = elem_bytes == 4 ? (lossy ? (const void*)decode_fused_kernel<uint32_t, true> : (const void*)decode_fused_kernel<uint32_t, false>)
+                         : (lossy ? (const void*)decode_fused_kernel<uint64_t, true> : (const void*)decode_fused_kernel<uint64_t, false>);
   } else {
     fn = elem_bytes == 4 ? (lossy ? encode_fn<uint32_t, true>(variant) : encode_fn<uint32_t, false>(variant))
                          : (lossy ? encode_fn<uint64_t, true>(variant) : encode_fn<uint64_t, false>(variant));

@@ -10,7 +10,7 @@ def fake_measure(costs, digests=None):
     def measure(cfg):
         calls.append(dict(cfg))
         e = 1.0 - 0.1 * (cfg["encode_variant"] == 3) - 0.05 * (cfg["lb_window"] == 8)
-        d = 1.0 - 0.2 * (cfg["decode_grid_mult"] == 2.0)
+        d = 1.0 - 0.2 * (cfg["decode_grid_mult"] == 2.0 and cfg["decoder"] == 0)
         dig = "same" if digests is None else digests(cfg)
         return e * costs, d * costs, dig
     return measure, calls

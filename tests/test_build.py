@@ -67,6 +67,12 @@ def test_barriers_in_single_outer_loop(asm):
     # the batched decoder inlines the chunk decoder once per (dtype, mode)
     bdec = [k for k in b if "batch_decode_kernel" in k]
     assert len(bdec) == 1 and b[bdec[0]] == [1] * 8, b.get(bdec[0] if bdec else None)
+    # the fused decoder: the first chunk's offset sum before the loop, then per
+    # chunk the payload cover, the stage reuse and the next offset window
+    fused = [k for k in b if "decode_fused_kernel" in k]
+    assert len(fused) == 4
+    for k in fused:
+        assert b[k] == [0, 1, 1, 1], (k, b[k])
     # single-wave workgroups (one-wave encoder, batched encoder) need no barrier instruction
     assert not [k for k in b if "wave" in k or "batch_encode" in k]
-    assert all(d == 1 for v in b.values() for d in v), b
+    assert all(d == 1 for k, v in b.items() if k not in fused for d in v), b

@@ -82,6 +82,8 @@ def test_sizes_and_messages():
         assert lib.sleek_gpu_set_lookback(g, w) == _lib.EINVAL, (g, w)
     for g, w in ((8, 1), (16, 64), (32, 7), (0, 0)):
         assert lib.sleek_gpu_set_lookback(g, w) == 0, (g, w)
+    assert lib.sleek_gpu_set_decoder(2) == _lib.EINVAL and lib.sleek_gpu_set_decoder(-1) == _lib.EINVAL
+    assert lib.sleek_gpu_set_decoder(1) == 0 and lib.sleek_gpu_set_decoder(0) == 0
 
 
 def test_argument_checks_before_any_device_work():

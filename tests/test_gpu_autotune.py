@@ -21,7 +21,7 @@ def test_tune_small_field(gpu, tmp_path):
     x = torch.from_numpy(a).cuda()
     try:
         best, trace = autotune.tune(x, 1e-3, reps=2, cache=True, path=str(tmp_path / "t.json"))
-        n_cand = 1 + (len(autotune.ENCODE_SHAPES) - 1) + len(autotune.LOOKBACK) + len(autotune.DECODE_GRIDS)
+        n_cand = 1 + (len(autotune.ENCODE_SHAPES) - 1) + len(autotune.LOOKBACK) + len(autotune.DECODE_SHAPES)
         assert len(trace) >= n_cand - 2
         assert set(best) == set(autotune.DEFAULT)
         # cached: a second call applies without searching

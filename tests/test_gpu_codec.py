@@ -129,6 +129,33 @@ def test_encoder_variant_invariance(gpu, dt, variant):
         lib.sleek_gpu_set_tuning(0, 0, 0)
 
 
+@pytest.mark.parametrize("variant", [0, 1])
+def test_decoder_variant_invariance(gpu, variant):
+    """Both decoders (sleek_gpu_set_decoder), at the automatic grid and at 1 and 7
+    workgroups (the fused decoder's offset windows then span many chunks)."""
+    say(f"decoder variant={variant}")
+    from sleek_amd import _lib
+    from sleek_amd.datagen import exact_field, inject_specials
+
+    lib = _lib.load()
+    try:
+        _lib.check(lib.sleek_gpu_set_decoder(variant), "decoder")
+        for dt, n in ((np.float32, 4096 * 37 + 5), (np.float64, 2048 * 21), (np.float32, 3)):
+            a = exact_field(n, 31, dt)
+            inject_specials(a, 1e-3, 5)
+            for eb in (None, 1e-3):
+                ref = po.compress(a, eb)
+                od, rc = po.decompress(ref, dt, eb)
+                assert rc == 0
+                for grid in (0, 1, 7):
+                    _lib.check(lib.sleek_gpu_set_tuning(0, 0, grid), "tuning")
+                    _, d = gpu_roundtrip(a, eb)
+                    assert d.tobytes() == od.tobytes(), (np.dtype(dt).name, n, eb, grid)
+    finally:
+        lib.sleek_gpu_set_decoder(0)
+        lib.sleek_gpu_set_tuning(0, 0, 0)
+
+
 @pytest.mark.parametrize("variant", [1, 3, 4])
 def test_lookback_geometry_invariance(gpu, variant):
     """Look-back group sizes 8..64 and initial windows 1..64 (sleek_gpu_set_lookback)

@@ -105,7 +105,7 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
  * default (currently 4), 1 = pipelined 256-thread encoder at a
  * 4-waves-per-SIMD register budget, 2 = the same at the compiler's register
  * choice, 3 = one wave per chunk, 4 = one wave per chunk pipelined over the
- * wave's chunks;
+ * wave's chunks, 5 = two waves per chunk pipelined the same way;
  * encode_grid / decode_grid: grid sizes (0 = automatic).  They
  * change speed only, never the bytes.  Applies to calls made after it
  * returns; SLEEK_EINVAL for an unknown variant or a negative size. */

@@ -100,7 +100,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map, long long tiles, uint64_t* ws,
     int lb_shift, int lb_window)
 {
-  __shared__ uint32_t stage[kStageBytes / 4];
+  __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   long long t = blockIdx.x;
   if (t >= tiles) return;
   EncTile ec = enc_tile(items, tile_map, t, ws, lb_shift, lb_window);
@@ -159,7 +159,7 @@ __global__ __launch_bounds__(kScanThreads) void batch_offsets_kernel(const Batch
 __global__ __launch_bounds__(kThreads) void batch_decode_kernel(const BatchItem* items, const uint32_t* tile_map,
                                                                 long long tiles, uint64_t* ws)
 {
-  __shared__ uint32_t stage[kStageBytes / 4];
+  __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   for (long long t = blockIdx.x; t < tiles; t += gridDim.x) {
     const BatchItem* it = items + uni(tile_map[t]);
     const DecodeArgs a = decode_args(it, ws);

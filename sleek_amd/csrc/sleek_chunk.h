@@ -404,6 +404,159 @@ __device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const Sca
   finish_chunk<W, LOSSY>(a, st, c, nchunks, sh.csize, stage);
 }
 
+// ---- two-wave chunk coder ----
+// Wave h of a 128-thread workgroup holds subchunks 16h .. 16h+15 of the
+// chunk (slice l of each in lane l, 32 VGPRs per chunk), half the
+// registers of the one-wave coder.  The halves' packed bits meet at a
+// subchunk boundary, which is 8-byte aligned inside the payload, so each
+// wave clears and packs its own qwords of the staging buffer.
+constexpr int kHalfSub = kSub / 2;
+
+__device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int h, uint2 raw[kHalfSub])
+{
+  const int lane = threadIdx.x & 63;
+  const long long base = c * kChunk;
+  const int osize = (int)min((long long)kChunk, a.insize - base);
+  const uint2* const p = (const uint2*)(a.in + base + h * kHalfSub * kSubBytes) + lane;
+  if (osize == kChunk) {
+#pragma unroll
+    for (int s = 0; s < kHalfSub; s++) raw[s] = p[s * (kSubBytes / 8)];
+  } else {
+    int rem = osize - h * kHalfSub * kSubBytes - lane * 8;   // opaque: see load_raw
+    asm volatile("" : "+v"(rem));
+    const uint2* const safe = (const uint2*)(a.in + base);
+#pragma unroll
+    for (int s = 0; s < kHalfSub; s++) raw[s] = *(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
+  }
+}
+
+// Codes of the wave's half in place (tail zeroed first); returns the half's
+// 16 bit-length bytes.
+template <typename W, bool LOSSY>
+__device__ __forceinline__ void shape_half(const EncodeArgs& a, long long c, int h, uint2 slot[kHalfSub], uint32_t Lh[4])
+{
+  constexpr int PL = Word<W>::PerLane;
+  const W offs = (W)a.offs;
+  const int osize = (int)min((long long)kChunk, a.insize - c * kChunk);
+  if (osize < kChunk) {
+    const int rem = osize - h * kHalfSub * kSubBytes - (int)(threadIdx.x & 63) * 8;
+#pragma unroll
+    for (int s = 0; s < kHalfSub; s++) {
+      const int r = rem - s * kSubBytes;
+      slot[s].x = r > 0 ? slot[s].x : 0u;
+      slot[s].y = r >= 8 ? slot[s].y : 0u;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++) Lh[i] = 0;
+#pragma unroll
+  for (int s = 0; s < kHalfSub; s++) {
+    W v[PL];
+    lane_values(slot[s], v);
+    W orv = 0;
+#pragma unroll
+    for (int k = 0; k < PL; k++) {
+      v[k] = to_code<W, LOSSY>(v[k], a.eb_e, a.thr_e, offs);
+      orv |= v[k];
+    }
+    slot[s] = values_lane(v);
+    Lh[s / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (s % 4));
+  }
+}
+
+// The chunk's shape from its 32 bit-length bytes (exchanged through LDS).
+template <typename W>
+__device__ __forceinline__ ChunkShape shape_of(const EncodeArgs& a, long long c, const uint32_t* xl)
+{
+  ChunkShape sh;
+  sh.osize = (int)min((long long)kChunk, a.insize - c * kChunk);
+  uint32_t nb = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    sh.L[i] = uni(xl[i]);
+    nb += byte_sum(sh.L[i]);
+  }
+  sh.total_bits = (int)nb * Word<W>::PerSub;
+  const int newsize = kSub + 2 + sh.total_bits / 8;
+  sh.csize = newsize < sh.osize ? newsize : sh.osize;
+  return sh;
+}
+
+// The wave's half of the payload into the shared staging buffer: its own
+// qwords cleared, then its fields OR-ed in; wave 0 writes the bit-length
+// bytes, wave 1 the trailer.  Raw chunks: each wave its own half of the
+// bytes.  No barrier inside: the waves' regions are disjoint.
+template <typename W, bool LOSSY>
+__device__ __forceinline__ void pack_half(const EncodeArgs& a, long long c, int h, const uint2 code[kHalfSub],
+                                          const ChunkShape& sh, uint32_t* stage)
+{
+  constexpr int PL = Word<W>::PerLane;
+  const int lane = threadIdx.x & 63;
+  if (sh.csize < sh.osize) {
+    const int mid = 8 * kSub + sub_bit_offset<W>(sh.L, kHalfSub);   // first bit of subchunk 16
+    const int q0 = h == 0 ? 4 : mid / 64;
+    const int q1 = h == 0 ? mid / 64 : (8 * kSub + sh.total_bits) / 64 + 1;   // wave 1: through the trailer
+    for (int q = q0 + lane; q < q1; q += 64) ((uint2*)stage)[q] = make_uint2(0u, 0u);
+    if (h == 0 && lane == 0) {
+      ((uint4*)stage)[0] = make_uint4(sh.L[0], sh.L[1], sh.L[2], sh.L[3]);
+      ((uint4*)stage)[1] = make_uint4(sh.L[4], sh.L[5], sh.L[6], sh.L[7]);
+    }
+    uint32_t lw[4];   // this half's bit lengths (h is uniform: no per-lane indexing)
+#pragma unroll
+    for (int i = 0; i < 4; i++) lw[i] = h == 0 ? sh.L[i] : sh.L[4 + i];
+    int p0 = h == 0 ? 8 * kSub : mid;
+#pragma unroll
+    for (int s = 0; s < kHalfSub; s++) {
+      const int ln = sub_length(lw, s);
+      if (ln) {
+        W v[PL];
+        lane_values(code[s], v);
+        or_field(stage, p0 + lane * PL * ln, lane_field(v, ln));
+      }
+      p0 += ln * Word<W>::PerSub;
+    }
+    if (h == 1 && lane == 0) stage[8 + sh.total_bits / 32] = (uint32_t)sh.osize;   // after the wave's ORs
+  } else if constexpr (LOSSY) {
+    const uint2* const safe = (const uint2*)(a.in + c * kChunk);
+    const uint2* const p = safe + h * kHalfSub * (kSubBytes / 8) + lane;
+    const int rem = sh.osize - h * kHalfSub * kSubBytes - lane * 8;
+#pragma unroll 4
+    for (int s = 0; s < kHalfSub; s++)
+      ((uint2*)stage)[lane + (h * kHalfSub + s) * (kSubBytes / 8)] = load_lane_rem(p, safe, s, kSubBytes / 8, rem);
+  } else {
+#pragma unroll
+    for (int s = 0; s < kHalfSub; s++) {
+      W v[PL];
+      lane_values(code[s], v);
+#pragma unroll
+      for (int k = 0; k < PL; k++) v[k] = lossless_value<W>(v[k]);
+      ((uint2*)stage)[lane + (h * kHalfSub + s) * (kSubBytes / 8)] = values_lane(v);
+    }
+  }
+}
+
+// len bytes of the staging buffer to dst (any byte address) by both waves:
+// 16-byte lines alternate between the waves, wave 0 stores the head bytes
+// before the first 16-byte boundary, wave 1 the tail bytes.
+__device__ __forceinline__ void pair_store(uint8_t* dst, const uint32_t* lds, int len, int h)
+{
+  const int lane = threadIdx.x & 63;
+  const int mis = (int)((uintptr_t)dst & 15);
+  const int head = min(len, (16 - mis) & 15);
+  const int nfull = (len - head) >> 4;
+  const int tail = len - head - (nfull << 4);
+  const uint8_t* lb = (const uint8_t*)lds;
+  if (h == 0 && lane < head) dst[lane] = lb[lane];
+  if (h == 1 && lane < tail) {
+    const int i = head + (nfull << 4) + lane;
+    dst[i] = lb[i];
+  }
+  for (int j = h * 64 + lane; j < nfull; j += 128) {
+    const int p = head + (j << 4);
+    *(uint4*)(dst + p) = lds_read16(lds, p);
+  }
+}
+
 // Validity of a stream header against the caller's limits (uniform).
 __device__ __forceinline__ bool header_ok(const DecodeArgs& a, long long hdr, long long* nchunks, uint64_t* why)
 {

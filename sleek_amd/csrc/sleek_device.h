@@ -373,13 +373,30 @@ __device__ __forceinline__ void publish_inclusive(const ScanState& s, long long 
   if (((c + 1) & ((1ll << s.gshift) - 1)) == 0 || c == nchunks - 1) flag_max(s.group + g, kInc | incl);
 }
 
-// 16 bytes starting at an arbitrary byte offset of a dword-aligned LDS buffer
-// (reads up to 4 bytes past the slice).
+// 16 bytes starting at an arbitrary byte offset of a 16-byte-aligned LDS
+// buffer.  byte_off must be congruent modulo 16 across the wave's active
+// lanes (callers step it by 16 per lane), so the two aligned 16-byte lines
+// read per lane are consecutive across lanes (no bank conflicts; five
+// dword reads per lane 16 bytes apart would conflict 8-way) and the word
+// selection and bit shift are wave-uniform.  Reads up to 32 bytes from the
+// line holding byte_off.
 __device__ __forceinline__ uint4 lds_read16(const uint32_t* lds, int byte_off)
 {
-  const int q = byte_off >> 2;
-  const uint32_t sh = (uint32_t)(byte_off & 3) * 8u;
-  const uint32_t x0 = lds[q], x1 = lds[q + 1], x2 = lds[q + 2], x3 = lds[q + 3], x4 = lds[q + 4];
+  const int sub = uni(byte_off & 15);
+  const uint32_t sh = (uint32_t)(sub & 3) * 8u;
+  const int line = byte_off >> 4;
+  const uint4 a = ((const uint4*)lds)[line];
+  const uint4 b = ((const uint4*)lds)[line + 1];
+  uint32_t x0, x1, x2, x3, x4;
+  if (sub < 4) {
+    x0 = a.x; x1 = a.y; x2 = a.z; x3 = a.w; x4 = b.x;
+  } else if (sub < 8) {
+    x0 = a.y; x1 = a.z; x2 = a.w; x3 = b.x; x4 = b.y;
+  } else if (sub < 12) {
+    x0 = a.z; x1 = a.w; x2 = b.x; x3 = b.y; x4 = b.z;
+  } else {
+    x0 = a.w; x1 = b.x; x2 = b.y; x3 = b.z; x4 = b.w;
+  }
   return make_uint4(__builtin_amdgcn_alignbit(x1, x0, sh), __builtin_amdgcn_alignbit(x2, x1, sh),
                     __builtin_amdgcn_alignbit(x3, x2, sh), __builtin_amdgcn_alignbit(x4, x3, sh));
 }

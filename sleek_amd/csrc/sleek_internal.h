@@ -37,7 +37,8 @@ struct DecodeArgs {
 // pipeline, register budget of 4 waves per SIMD (the most its two LDS
 // staging buffers allow); kEncodeFree: the same kernel at the compiler's
 // register choice; kEncodeWave: one 64-thread wave per chunk; kEncodeWavePipe:
-// one wave per chunk, pipelined over the wave's chunks.  kEncodeDefault
+// one wave per chunk, pipelined over the wave's chunks; kEncodePair: two
+// waves per chunk (128 threads), pipelined the same way.  kEncodeDefault
 // resolves to the fastest measured (kEncodeWavePipe).  Output bytes never
 // depend on the variant or on the grid size.
 constexpr int kEncodeDefault = 0;
@@ -45,7 +46,8 @@ constexpr int kEncodePipelined = 1;
 constexpr int kEncodeFree = 2;
 constexpr int kEncodeWave = 3;
 constexpr int kEncodeWavePipe = 4;
-constexpr int kEncodeVariants = 5;
+constexpr int kEncodePair = 5;
+constexpr int kEncodeVariants = 6;
 inline int resolve_variant(int v) { return v == kEncodeDefault ? kEncodeWavePipe : v; }
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);

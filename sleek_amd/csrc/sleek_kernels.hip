@@ -33,7 +33,7 @@ template <typename W, bool LOSSY, int WAVES>
 __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES, 4))) void encode_kernel(EncodeArgs a)
 {
   constexpr int PL = Word<W>::PerLane;
-  __shared__ uint32_t stage[2][kStageBytes / 4];
+  __shared__ __align__(16) uint32_t stage[2][kStageBytes / 4];
   __shared__ uint32_t lgs[8];
   __shared__ unsigned long long s_excl;
 
@@ -202,7 +202,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void encode_wave_kernel(EncodeArgs a)
 {
-  __shared__ uint32_t stage[kStageBytes / 4];
+  __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
   if (nchunks == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void encode_wave2_kernel(EncodeArgs a)
 {
-  __shared__ uint32_t stage[kStageBytes / 4];
+  __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
   if (nchunks == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -264,6 +264,84 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   }
 }
 
+// Encoder, two waves per chunk, pipelined over the workgroup's chunks like
+// encode_wave2_kernel: half the registers per wave (four waves per SIMD).
+// Iteration for chunk c (codes and shape made in the previous iteration):
+//   both waves: issue their half of chunk n's loads, pack their half of c;
+//   code their half of n, post its bit lengths -> barrier -> n's shape;
+//   wave 0: publish n's size, look back for c -> barrier -> both store
+//   their lines of c -> barrier (staging buffer free).
+template <typename W, bool LOSSY>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) void encode_pair_kernel(EncodeArgs a)
+{
+  __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
+  __shared__ uint32_t xl[2][8];
+  __shared__ unsigned long long xexcl;
+  const int h = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  const long long nchunks = (a.insize + kChunk - 1) / kChunk;
+  if (nchunks == 0) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      *(long long*)a.out = 0;
+      *a.outsize = 8;
+    }
+    return;
+  }
+  const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
+  const long long data_base = 8 + 2 * nchunks;
+  long long c = blockIdx.x;
+  if (c >= nchunks) return;
+  uint2 cur[kHalfSub];
+  uint32_t lh[4];
+  load_half(a, c, h, cur);
+  shape_half<W, LOSSY>(a, c, h, cur, lh);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) xl[0][4 * h + i] = lh[i];
+  }
+  __syncthreads();
+  ChunkShape sc = shape_of<W>(a, c, xl[0]);
+  if (h == 0) publish_chunk(a, st, c, nchunks, sc.csize);
+  int par = 1;
+  while (true) {
+    const long long n = c + gridDim.x;
+    const bool more = n < nchunks;
+    uint2 nxt[kHalfSub];
+    if (more) load_half(a, n, h, nxt);
+    pack_half<W, LOSSY>(a, c, h, cur, sc, stage);
+    if (more) {
+      shape_half<W, LOSSY>(a, n, h, nxt, lh);
+      if (lane == 0) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) xl[par][4 * h + i] = lh[i];
+      }
+    }
+    __syncthreads();   // both halves packed; n's bit lengths posted
+    ChunkShape sn;
+    if (more) sn = shape_of<W>(a, n, xl[par]);
+    if (h == 0) {
+      if (more) publish_chunk(a, st, n, nchunks, sn.csize);
+      const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
+      if (lane == 0) {
+        if (c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)sc.csize);
+        xexcl = excl;
+        if (c == 0) *(long long*)a.out = a.insize;
+        if (c == nchunks - 1) *a.outsize = data_base + (long long)excl + sc.csize;
+      }
+    }
+    __syncthreads();   // c's stream offset known
+    const long long excl = (long long)uni((long long)xexcl);
+    pair_store(a.out + data_base + excl, stage, sc.csize, h);
+    __syncthreads();   // staging buffer read before it is packed again
+    if (!more) break;
+    c = n;
+    sc = sn;
+    par ^= 1;
+#pragma unroll
+    for (int s = 0; s < kHalfSub; s++) cur[s] = nxt[s];
+  }
+}
+
 __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
 {
   __shared__ long long wsum[kScanThreads / 64];
@@ -281,7 +359,7 @@ __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(kThreads) void decode_fused_kernel(DecodeArgs a)
 {
-  __shared__ uint32_t stage[kStageBytes / 4];
+  __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   __shared__ long long red[2][kThreads / 64];
   const long long hdr = uni(*(const long long*)a.in);
   long long nchunks = 0;
@@ -314,7 +392,7 @@ __global__ __launch_bounds__(kThreads) void decode_fused_kernel(DecodeArgs a)
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 {
-  __shared__ uint32_t stage[kStageBytes / 4];
+  __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   const long long hdr = uni(*(const long long*)a.in);
   long long nchunks = 0;
   uint64_t why = 0;
@@ -330,6 +408,7 @@ static hipError_t launch_encode(const EncodeArgs& a, int grid, int variant, hipS
   variant = resolve_variant(variant);
   if (variant == kEncodeWave) hipLaunchKernelGGL((encode_wave_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
   else if (variant == kEncodeWavePipe) hipLaunchKernelGGL((encode_wave2_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
+  else if (variant == kEncodePair) hipLaunchKernelGGL((encode_pair_kernel<W, LOSSY>), dim3(grid), dim3(128), 0, stream, a);
   else if (variant == kEncodeFree) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 1>), dim3(grid), dim3(kThreads), 0, stream, a);
   else hipLaunchKernelGGL((encode_kernel<W, LOSSY, 4>), dim3(grid), dim3(kThreads), 0, stream, a);
   return hipGetLastError();
@@ -367,6 +446,7 @@ static const void* encode_fn(int variant)
   variant = resolve_variant(variant);
   if (variant == kEncodeWave) return (const void*)encode_wave_kernel<W, LOSSY>;
   if (variant == kEncodeWavePipe) return (const void*)encode_wave2_kernel<W, LOSSY>;
+  if (variant == kEncodePair) return (const void*)encode_pair_kernel<W, LOSSY>;
   return variant == kEncodeFree ? (const void*)encode_kernel<W, LOSSY, 1> : (const void*)encode_kernel<W, LOSSY, 4>;
 }
 
@@ -385,7 +465,7 @@ int resident_grid(int elem_bytes, bool lossy, bool decode, int variant)
                          : (lossy ? encode_fn<uint64_t, true>(variant) : encode_fn<uint64_t, false>(variant));
   }
   const int rv = resolve_variant(variant);
-  const int threads = (!decode && (rv == kEncodeWave || rv == kEncodeWavePipe)) ? 64 : kThreads;
+  const int threads = decode ? kThreads : ((rv == kEncodeWave || rv == kEncodeWavePipe) ? 64 : (rv == kEncodePair ? 128 : kThreads));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);
 }

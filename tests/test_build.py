@@ -73,6 +73,13 @@ def test_barriers_in_single_outer_loop(asm):
     assert len(fused) == 4
     for k in fused:
         assert b[k] == [0, 1, 1, 1], (k, b[k])
+    # the two-wave encoder: the first chunk's bit-length exchange before the loop,
+    # then per chunk the exchange, the stream offset and the staging buffer's reuse
+    pair = [k for k in b if "encode_pair_kernel" in k]
+    assert len(pair) == 4
+    for k in pair:
+        assert b[k] == [0, 1, 1, 1], (k, b[k])
+    fused += pair
     # single-wave workgroups (one-wave encoder, batched encoder) need no barrier instruction
     assert not [k for k in b if "wave" in k or "batch_encode" in k]
     assert all(d == 1 for k, v in b.items() if k not in fused for d in v), b

@@ -87,10 +87,14 @@ __device__ __forceinline__ void enc_pack(const EncTile& e, const uint2 slot[kSub
 __device__ __forceinline__ void enc_finish(const EncTile& e, int csize, const uint32_t* stage)
 {
   // the element type only matters to a look-back that sizes a missing chunk itself
-  if (e.kind == 0) finish_chunk<uint32_t, false>(e.a, e.st, e.c, e.nchunks, csize, stage);
-  else if (e.kind == 1) finish_chunk<uint32_t, true>(e.a, e.st, e.c, e.nchunks, csize, stage);
-  else if (e.kind == 2) finish_chunk<uint64_t, false>(e.a, e.st, e.c, e.nchunks, csize, stage);
-  else finish_chunk<uint64_t, true>(e.a, e.st, e.c, e.nchunks, csize, stage);
+  uint64_t excl = 0;
+  if (e.c != 0) {
+    if (e.kind == 0) excl = look_back<uint32_t, false>(e.a, e.st, e.c, e.nchunks);
+    else if (e.kind == 1) excl = look_back<uint32_t, true>(e.a, e.st, e.c, e.nchunks);
+    else if (e.kind == 2) excl = look_back<uint64_t, false>(e.a, e.st, e.c, e.nchunks);
+    else excl = look_back<uint64_t, true>(e.a, e.st, e.c, e.nchunks);
+  }
+  finish_chunk(e.a, e.st, e.c, e.nchunks, csize, stage, excl);
 }
 
 // Encoder: one wave per tile, pipelined over the wave's tiles like the

@@ -251,7 +251,10 @@ __device__ __forceinline__ void mask_tail(const EncodeArgs& a, long long c, uint
 {
   const int osize = (int)min((long long)kChunk, a.insize - c * kChunk);
   if (osize < kChunk) {
-    const int rem = osize - (int)(threadIdx.x & 63) * 8;
+    // opaque, so the compiler derives the slices' limits from it here rather
+    // than keeping 32 per-lane constants live across the whole kernel
+    int rem = osize - (int)(threadIdx.x & 63) * 8;
+    asm volatile("" : "+v"(rem));
 #pragma unroll
     for (int s = 0; s < kSub; s++) {
       const int r = rem - s * kSubBytes;
@@ -280,8 +283,7 @@ __device__ __forceinline__ ChunkShape shape_codes(const EncodeArgs& a, long long
   ChunkShape sh;
   sh.osize = (int)min((long long)kChunk, a.insize - c * kChunk);
   mask_tail(a, c, slot);
-#pragma unroll
-  for (int i = 0; i < 8; i++) sh.L[i] = 0;
+  uint32_t ov[kSub];
 #pragma unroll
   for (int s = 0; s < kSub; s++) {
     W v[PL];
@@ -293,8 +295,9 @@ __device__ __forceinline__ ChunkShape shape_codes(const EncodeArgs& a, long long
       orv |= v[k];
     }
     slot[s] = values_lane(v);
-    sh.L[s / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (s % 4));
+    ov[s] = lane_length_word(orv);
   }
+  wave_lengths<kSub, sizeof(W) == 8>(ov, sh.L);
   uint32_t nb = 0;
 #pragma unroll
   for (int i = 0; i < 8; i++) nb += byte_sum(sh.L[i]);
@@ -373,15 +376,13 @@ __device__ __forceinline__ void pack_codes(const EncodeArgs& a, long long c, con
   __syncthreads();
 }
 
-// Look-back for chunk c, its inclusive word, and the store of its payload
-// (plus the header for chunk 0 and *outsize for the last chunk).
-template <typename W, bool LOSSY>
+// Chunk c's inclusive word (excl: its exclusive offset) and the store of its
+// payload (plus the header for chunk 0 and *outsize for the last chunk).
 __device__ __forceinline__ void finish_chunk(const EncodeArgs& a, const ScanState& st, long long c, long long nchunks,
-                                             int csize, const uint32_t* stage)
+                                             int csize, const uint32_t* stage, uint64_t excl)
 {
   const int lane = threadIdx.x & 63;
   const long long data_base = 8 + 2 * nchunks;
-  const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
   if (lane == 0 && c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)csize);
   wave_store_unaligned(a.out + data_base + (long long)excl, stage, csize);
   if (lane == 0) {
@@ -401,7 +402,7 @@ __device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const Sca
   const ChunkShape sh = shape_codes<W, LOSSY>(a, c, slot);
   publish_chunk(a, st, c, nchunks, sh.csize);
   pack_codes<W, LOSSY>(a, c, slot, sh, stage);
-  finish_chunk<W, LOSSY>(a, st, c, nchunks, sh.csize, stage);
+  finish_chunk(a, st, c, nchunks, sh.csize, stage, (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks));
 }
 
 // ---- two-wave chunk coder ----
@@ -447,8 +448,7 @@ __device__ __forceinline__ void shape_half(const EncodeArgs& a, long long c, int
       slot[s].y = r >= 8 ? slot[s].y : 0u;
     }
   }
-#pragma unroll
-  for (int i = 0; i < 4; i++) Lh[i] = 0;
+  uint32_t ov[kHalfSub];
 #pragma unroll
   for (int s = 0; s < kHalfSub; s++) {
     W v[PL];
@@ -460,8 +460,9 @@ __device__ __forceinline__ void shape_half(const EncodeArgs& a, long long c, int
       orv |= v[k];
     }
     slot[s] = values_lane(v);
-    Lh[s / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (s % 4));
+    ov[s] = lane_length_word(orv);
   }
+  wave_lengths<kHalfSub, sizeof(W) == 8>(ov, Lh);
 }
 
 // The chunk's shape from its 32 bit-length bytes (exchanged through LDS).

@@ -206,6 +206,90 @@ __device__ __forceinline__ int wave_bit_length(uint64_t orv)
                   max((uint32_t)__builtin_amdgcn_readlane((int)b, 32), (uint32_t)__builtin_amdgcn_readlane((int)b, 48)));
 }
 
+// ---- bit lengths of all subchunks at once ----
+// The per-lane values of N subchunks (32 for a chunk, 16 for a half chunk):
+// lane l's OR of its codes of subchunk s, or (MAX) their bit length.  They
+// are combined over the wave by a transposing butterfly: in each exchange a
+// lane keeps half of its values and sends the other half to a partner lane
+// that keeps the opposite half, so a lane's values halve while the lanes
+// they cover double (row_mirror, row_half_mirror and two quad_perms inside
+// rows of 16, then an xor-16 swizzle, then an xor-32 permute).  N - 1
+// exchanges cover all N subchunks; reducing each subchunk on its own costs
+// 4 DPP steps and 4 lane reads per subchunk.  Afterwards subchunk s's value
+// is in lane l for s = (l3 l2 l1 l0 l4) in binary when N = 32 (l_k: bit k
+// of l) and s = (l3 l2 l1 l0) when N = 16.
+template <bool MAX>
+__device__ __forceinline__ uint32_t bfly_op(uint32_t a, uint32_t b)
+{
+  return MAX ? max(a, b) : (a | b);
+}
+
+template <int H, int BIT, int CTRL, bool MAX, int N>
+__device__ __forceinline__ void bfly_step(uint32_t (&v)[N], int lane)
+{
+  const bool hi = (lane >> BIT) & 1;
+#pragma unroll
+  for (int i = 0; i < H; i++) {
+    const uint32_t keep = hi ? v[i + H] : v[i];
+    const uint32_t send = hi ? v[i] : v[i + H];
+    v[i] = bfly_op<MAX>(keep, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)send, CTRL, 0xF, 0xF, false));
+  }
+}
+
+__device__ __forceinline__ uint32_t swz_xor16(uint32_t v) { return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F); }
+
+// The N subchunks' bit lengths, byte-packed little-endian into N/4 uniform
+// dwords (L[i] byte j: subchunk 4i + j).  v is consumed.
+template <int N, bool MAX>
+__device__ __forceinline__ void wave_lengths(uint32_t (&v)[N], uint32_t* L)
+{
+  static_assert(N == 32 || N == 16, "a chunk or half a chunk");
+  const int lane = threadIdx.x & 63;
+  if constexpr (N == 32) {
+    bfly_step<16, 3, 0x140, MAX>(v, lane);   // row_mirror
+    bfly_step<8, 2, 0x141, MAX>(v, lane);    // row_half_mirror
+    bfly_step<4, 1, 0x4E, MAX>(v, lane);     // quad_perm [2,3,0,1]
+    bfly_step<2, 0, 0xB1, MAX>(v, lane);     // quad_perm [1,0,3,2]
+    const bool hi = (lane >> 4) & 1;
+    const uint32_t keep = hi ? v[1] : v[0];
+    const uint32_t send = hi ? v[0] : v[1];
+    v[0] = bfly_op<MAX>(keep, swz_xor16(send));
+  } else {
+    bfly_step<8, 3, 0x140, MAX>(v, lane);
+    bfly_step<4, 2, 0x141, MAX>(v, lane);
+    bfly_step<2, 1, 0x4E, MAX>(v, lane);
+    bfly_step<1, 0, 0xB1, MAX>(v, lane);
+    v[0] = bfly_op<MAX>(v[0], swz_xor16(v[0]));
+  }
+  const uint32_t r = bfly_op<MAX>(v[0], (uint32_t)__builtin_amdgcn_ds_bpermute((lane ^ 32) << 2, (int)v[0]));
+  const uint32_t bl = MAX ? r : 32u - (uint32_t)__clz(r);   // __clz(0) = 32
+  if constexpr (N == 32) {
+    // lanes l, l^1, l^16, l^17 hold bytes 0..3 of one dword: byte j = (l0 l4)
+    uint32_t t = bl << (8 * (((lane & 1) << 1) | ((lane >> 4) & 1)));
+    t |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0xB1, 0xF, 0xF, false);
+    t |= swz_xor16(t);
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+      L[i] = (uint32_t)__builtin_amdgcn_readlane((int)t, (((i >> 2) & 1) << 3) | (((i >> 1) & 1) << 2) | ((i & 1) << 1));
+  } else {
+    // lanes l .. l^3 hold bytes 0..3 of one dword: byte j = (l1 l0)
+    uint32_t t = bl << (8 * (lane & 3));
+    t |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0xB1, 0xF, 0xF, false);
+    t |= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)t, 0x4E, 0xF, 0xF, false);
+#pragma unroll
+    for (int i = 0; i < 4; i++) L[i] = (uint32_t)__builtin_amdgcn_readlane((int)t, ((i >> 1) << 3) | ((i & 1) << 2));
+  }
+}
+
+// A lane's contribution to its subchunk's bit length: the OR of its codes
+// (32-bit words), or the bit length of that OR (64-bit words, combined by MAX).
+__device__ __forceinline__ uint32_t lane_length_word(uint32_t orv) { return orv; }
+__device__ __forceinline__ uint32_t lane_length_word(uint64_t orv)
+{
+  const uint32_t hi = (uint32_t)(orv >> 32), lo = (uint32_t)orv;
+  return hi ? 64u - (uint32_t)__clz(hi) : 32u - (uint32_t)__clz(lo);
+}
+
 __device__ __forceinline__ int bit_length(uint32_t x) { return 32 - __clz(x); }
 __device__ __forceinline__ int bit_length(uint64_t x) { return 64 - __clzll(x); }
 

@@ -255,7 +255,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
       sn = shape_codes<W, LOSSY>(a, n, nxt);
       publish_chunk(a, st, n, nchunks, sn.csize);
     }
-    finish_chunk<W, LOSSY>(a, st, c, nchunks, sc.csize, stage);
+    finish_chunk(a, st, c, nchunks, sc.csize, stage, (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks));
     if (!more) break;
     c = n;
     sc = sn;

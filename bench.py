@@ -469,19 +469,20 @@ def main():
         _lib.check(_lib.load().sleek_gpu_set_tuning(args.encode_variant, args.encode_grid, args.decode_grid), "tuning")
         tuning = {"encode_variant": args.encode_variant, "encode_grid": args.encode_grid,
                   "decode_grid": args.decode_grid, "source": "command line"}
-    elif not args.no_autotune:
-        # launch shape for this input, searched before the timed region (bytes are checked invariant)
-        from sleek_amd import autotune
-
-        best, trace = autotune.tune(x, None, cache=False)
-        tuning = dict(best, source="autotune", candidates=len(trace))
-        for cfg, te, td in trace:
-            print(f"autotune {json.dumps(cfg)} encode {te * 1e3:.4f} ms decode {td * 1e3:.4f} ms",
-                  file=sys.stderr, flush=True)
     nbytes = x.numel() * 4
     rng = finite_range(x)
     ebs = [None] + [float(np.float32(np.float32(rng) * np.float32(e))) for e in MODES[1:]]
     labels = ["lossless"] + [f"eb={m:g}*range" for m in MODES[1:]]
+    if tuning is None and not args.no_autotune:
+        # launch shape for this input and these modes, searched before the timed
+        # region (bytes are checked invariant)
+        from sleek_amd import autotune
+
+        best, trace = autotune.tune(x, ebs, reps=3, cache=False)
+        tuning = dict(best, source="autotune", candidates=len(trace))
+        for cfg, te, td in trace:
+            print(f"autotune {json.dumps(cfg)} encode {te * 1e3:.4f} ms decode {td * 1e3:.4f} ms (all modes)",
+                  file=sys.stderr, flush=True)
     per, enc_sizes, t_total, wall = run_modes(x, ebs, labels, args.steps, args.warmup)
     ms_step = t_total / args.steps * 1e3
     stream = torch.cuda.current_stream()

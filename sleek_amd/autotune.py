@@ -27,7 +27,7 @@ import os
 DEFAULT = {"encode_variant": 0, "encode_grid_mult": 0.0, "lb_group": 64, "lb_window": 4, "decoder": 0,
            "decode_grid_mult": 0.0}
 
-ENCODE_SHAPES = [(0, 0.0), (4, 0.75), (1, 0.0), (1, 0.75), (3, 0.0)]
+ENCODE_SHAPES = [(0, 0.0), (5, 0.0), (4, 0.75), (1, 0.0), (3, 0.0)]
 MIN_GAIN = 0.01   # a candidate must beat the best so far by 1 % (timing noise is about that)
 LOOKBACK = [(16, 4), (32, 4), (64, 2), (64, 8), (64, 16)]
 DECODE_SHAPES = [(0, 1.0), (0, 2.0), (1, 0.0)]   # (decoder, grid multiple; 0.0 = automatic)
@@ -43,7 +43,13 @@ def size_bucket(nbytes):
 
 
 def key(device_name, elem_bytes, lossy, nbytes):
-    return f"{device_name}|f{8 * elem_bytes}|{'lossy' if lossy else 'lossless'}|{size_bucket(nbytes)}"
+    """Cache key; lossy is a bool, or the number of lossy modes next to a lossless one
+    when a mix of modes was tuned together."""
+    if isinstance(lossy, bool) or lossy in (0, 1):
+        mode = "lossy" if lossy else "lossless"
+    else:
+        mode = f"mixed{int(lossy)}"
+    return f"{device_name}|f{8 * elem_bytes}|{mode}|{size_bucket(nbytes)}"
 
 
 def load_cache(path=None):
@@ -121,14 +127,19 @@ def apply(cfg):
 
 def tune(x, eb=None, reps=5, cache=True, path=None):
     """Tunes the single-array codec on the CUDA tensor x (a representative
-    input) and applies the result; returns (config, trace)."""
+    input) and applies the result; returns (config, trace).  eb: one bound
+    (None = lossless) or a list of them: a configuration is then timed over
+    all those modes together, so one choice serves the mix."""
     import torch
 
     from . import codec
 
+    ebs = list(eb) if isinstance(eb, (list, tuple)) else [eb]
     dev = x.device
     nbytes = x.numel() * x.element_size()
-    k = key(torch.cuda.get_device_name(dev), x.element_size(), eb is not None, nbytes)
+    nlossy = sum(e is not None for e in ebs)
+    k = key(torch.cuda.get_device_name(dev), x.element_size(),
+            (nlossy > 0) if len(ebs) == 1 else nlossy + 1, nbytes)
     if cache:
         c = load_cache(path)
         if k in c:
@@ -145,26 +156,29 @@ def tune(x, eb=None, reps=5, cache=True, path=None):
 
     def measure(cfg):
         apply(cfg)
-        te, td = [], []
-        for _ in range(reps + 1):
-            a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
-            a0.record(stream)
-            codec.compress(x, eb, out=out, encsize=enc, workspace=ws)
-            a1.record(stream)
-            codec.decompress(out, x.dtype, eb, out=dec, decsize=dsz, workspace=ws)
-            a2.record(stream)
-            torch.cuda.synchronize()
-            te.append(a0.elapsed_time(a1) / 1e3)
-            td.append(a1.elapsed_time(a2) / 1e3)
-        n = int(enc.item())
-        if not ref:
-            ref["n"], ref["out"], ref["dec"] = n, out[:n].clone(), dec.clone()
-            digest = "reference"
-        else:
-            same = n == ref["n"] and torch.equal(out[:n], ref["out"]) and \
-                torch.equal(dec.view(torch.uint8), ref["dec"].view(torch.uint8))
-            digest = "reference" if same else f"changed (stream length {n})"
-        return float(sorted(te[1:])[reps // 2]), float(sorted(td[1:])[reps // 2]), digest
+        te_tot, td_tot, same = 0.0, 0.0, True
+        for i, b in enumerate(ebs):
+            te, td = [], []
+            for _ in range(reps + 1):
+                a0, a1, a2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+                a0.record(stream)
+                codec.compress(x, b, out=out, encsize=enc, workspace=ws)
+                a1.record(stream)
+                codec.decompress(out, x.dtype, b, out=dec, decsize=dsz, workspace=ws)
+                a2.record(stream)
+                torch.cuda.synchronize()
+                te.append(a0.elapsed_time(a1) / 1e3)
+                td.append(a1.elapsed_time(a2) / 1e3)
+            te_tot += float(sorted(te[1:])[reps // 2])
+            td_tot += float(sorted(td[1:])[reps // 2])
+            n = int(enc.item())
+            if i not in ref:
+                ref[i] = (n, out[:n].clone(), dec.clone())
+            else:
+                rn, rout, rdec = ref[i]
+                same = same and n == rn and torch.equal(out[:n], rout) and \
+                    torch.equal(dec.view(torch.uint8), rdec.view(torch.uint8))
+        return te_tot, td_tot, "reference" if same else "changed"
 
     try:
         best, trace = search(measure)

@@ -70,10 +70,12 @@ __device__ __forceinline__ EncTile enc_tile(const BatchItem* __restrict__ items,
 
 __device__ __forceinline__ ChunkShape enc_shape(const EncTile& e, uint2 slot[kSub])
 {
-  if (e.kind == 0) return shape_codes<uint32_t, false>(e.a, e.c, slot);
-  if (e.kind == 1) return shape_codes<uint32_t, true>(e.a, e.c, slot);
-  if (e.kind == 2) return shape_codes<uint64_t, false>(e.a, e.c, slot);
-  return shape_codes<uint64_t, true>(e.a, e.c, slot);
+  // subchunk by subchunk: the four element kinds' code paths leave no
+  // registers for the butterfly's 32 inputs
+  if (e.kind == 0) return shape_codes<uint32_t, false, false>(e.a, e.c, slot);
+  if (e.kind == 1) return shape_codes<uint32_t, true, false>(e.a, e.c, slot);
+  if (e.kind == 2) return shape_codes<uint64_t, false, false>(e.a, e.c, slot);
+  return shape_codes<uint64_t, true, false>(e.a, e.c, slot);
 }
 
 __device__ __forceinline__ void enc_pack(const EncTile& e, const uint2 slot[kSub], const ChunkShape& sh, uint32_t* stage)

@@ -274,8 +274,10 @@ struct ChunkShape {
 };
 
 // Codes of the chunk's values, in place (slices as load_raw left them),
-// and the chunk's shape.
-template <typename W, bool LOSSY>
+// and the chunk's shape.  BFLY: the subchunks' bit lengths by one
+// transposing butterfly over all 32 (32 more VGPRs live at its start), else
+// subchunk by subchunk (for kernels with no registers to spare).
+template <typename W, bool LOSSY, bool BFLY = true>
 __device__ __forceinline__ ChunkShape shape_codes(const EncodeArgs& a, long long c, uint2 slot[kSub])
 {
   constexpr int PL = Word<W>::PerLane;
@@ -284,6 +286,8 @@ __device__ __forceinline__ ChunkShape shape_codes(const EncodeArgs& a, long long
   sh.osize = (int)min((long long)kChunk, a.insize - c * kChunk);
   mask_tail(a, c, slot);
   uint32_t ov[kSub];
+#pragma unroll
+  for (int i = 0; i < 8; i++) sh.L[i] = 0;
 #pragma unroll
   for (int s = 0; s < kSub; s++) {
     W v[PL];
@@ -295,9 +299,10 @@ __device__ __forceinline__ ChunkShape shape_codes(const EncodeArgs& a, long long
       orv |= v[k];
     }
     slot[s] = values_lane(v);
-    ov[s] = lane_length_word(orv);
+    if constexpr (BFLY) ov[s] = lane_length_word(orv);
+    else sh.L[s / 4] |= (uint32_t)wave_bit_length(orv) << (8 * (s % 4));
   }
-  wave_lengths<kSub, sizeof(W) == 8>(ov, sh.L);
+  if constexpr (BFLY) wave_lengths<kSub, sizeof(W) == 8>(ov, sh.L);
   uint32_t nb = 0;
 #pragma unroll
   for (int i = 0; i < 8; i++) nb += byte_sum(sh.L[i]);

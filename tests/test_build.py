@@ -83,3 +83,27 @@ def test_barriers_in_single_outer_loop(asm):
     # single-wave workgroups (one-wave encoder, batched encoder) need no barrier instruction
     assert not [k for k in b if "wave" in k or "batch_encode" in k]
     assert all(d == 1 for k, v in b.items() if k not in fused for d in v), b
+
+
+CLI_NAMES = [f"sleek_{p}_{d}_{m}" for p in ("single", "double") for d in ("compressor", "decompressor")
+             for m in ("lossless", "lossy")]
+
+
+def test_cli_drivers_build_and_print_usage():
+    """The command-line drivers compile against the drop-in headers with the host
+    compiler, and answer a wrong argument count with the reference's usage line and
+    exit status (-1) before touching a device."""
+    if shutil.which("g++") is None or not os.path.exists(os.path.join(ROOT, "sleek_amd", "libsleek_amd.so")):
+        pytest.skip("needs g++ and the built library")
+    subprocess.run(["make", "-j8", "-C", os.path.join(ROOT, "sleek_amd", "cli")], check=True, capture_output=True)
+    for sub in ("", "standalone"):
+        for name in CLI_NAMES:
+            exe = os.path.join(ROOT, "sleek_amd", "cli", "bin", sub, name)
+            r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+            assert r.returncode == 255, (name, r.returncode, r.stderr)
+            want = "compressed_file_name decompressed_file_name" if "decompressor" in name else \
+                "input_file_name compressed_file_name"
+            assert f"USAGE: {exe} {want}" in r.stdout
+    r = subprocess.run([os.path.join(ROOT, "sleek_amd", "cli", "bin", "GPUmemcpy")], capture_output=True, text=True,
+                       timeout=60)
+    assert r.returncode == 255 and "USAGE:" in r.stdout

@@ -45,6 +45,7 @@ def parse():
     p.add_argument("--no-autotune", action="store_true", help="keep the default launch shape")
     p.add_argument("--no-f64", action="store_true", help="skip the float64 workload")
     p.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    p.add_argument("--no-transfer", action="store_true", help="skip the compressing host<->device copies")
     p.add_argument("--batch-arrays", type=int, default=4096)
     p.add_argument("--batch-max-mb", type=float, default=64.0, help="largest array of the batched workload")
     p.add_argument("--batch-steps", type=int, default=3)
@@ -424,6 +425,52 @@ def memcpy_d2d(x, reps=10):
             "traffic_GBps": round(2 * nbytes / mtm / 1e9, 1), "hbm_frac": round(2 * nbytes / mtm / HBM_PEAK, 4)}
 
 
+def bench_transfer(x, eb_lossy, reps=3):
+    """Compressing copies of the field between device and pinned host memory
+    (sleek_amd.transfer: segments compressed on one stream while the previous
+    one crosses PCIe on another), next to plain pinned copies of the same
+    bytes (the reference's memcpy.cu:114-139 legs).  Effective GB/s =
+    insize / t; every round trip is checked."""
+    import torch
+
+    from sleek_amd import transfer
+
+    nbytes = x.numel() * x.element_size()
+    host = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+    flat = x.view(torch.uint8).view(-1)
+
+    def timed(fn):
+        ts = []
+        for _ in range(reps + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            r = fn()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts[1:])), r
+
+    t_d2h, _ = timed(lambda: host.copy_(flat, non_blocking=True))
+    dev_back = torch.empty_like(flat)
+    t_h2d, _ = timed(lambda: dev_back.copy_(host, non_blocking=True))
+    out = {"bytes": nbytes, "segment_bytes": transfer.DEFAULT_SEGMENT,
+           "plain_d2h_GBps": round(nbytes / t_d2h / 1e9, 2), "plain_h2d_GBps": round(nbytes / t_h2d / 1e9, 2),
+           "modes": []}
+    del dev_back
+    for label, eb in (("lossless", None), ("eb=1e-3*range", eb_lossy)):
+        t_off, cont = timed(lambda: transfer.offload(x, eb))
+        t_rel, y = timed(lambda: transfer.reload(cont, x.device))
+        if eb is None:
+            assert torch.equal(y.view(torch.uint8), x.view(torch.uint8)), "offload/reload round trip mismatch"
+        out["modes"].append({"mode": label, "container_bytes": int(cont.numel()),
+                             "ratio": round(nbytes / cont.numel(), 4),
+                             "offload_ms": round(t_off * 1e3, 3), "reload_ms": round(t_rel * 1e3, 3),
+                             "offload_GBps": round(nbytes / t_off / 1e9, 2),
+                             "reload_GBps": round(nbytes / t_rel / 1e9, 2),
+                             "vs_plain_d2h": round(t_d2h / t_off, 3), "vs_plain_h2d": round(t_h2d / t_rel, 3)})
+        del cont, y
+    return out
+
+
 def bench_f64(args, dev):
     """The float64 configuration: a 384^3 field (12 sinusoids + lognormal
     noise, 0.01 % specials), lossless and eb = {1e-6, 1e-9} x range, next to
@@ -525,6 +572,8 @@ def main():
         "memcpy_d2d": mc,
         "wall_s": round(wall, 3),
     }
+    if not args.no_transfer:
+        out["transfer"] = bench_transfer(x, ebs[2])
     x_host = None if args.no_cpu else x.cpu().numpy()
     del x
     torch.cuda.empty_cache()

@@ -1,0 +1,51 @@
+"""Compressing device<->host copies: offload() leaves a container in pinned host
+memory whose per-segment streams are the oracle's streams of those segments,
+and reload() brings back the oracle's decode (the input itself when lossless)."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("dt,eb,n,seg", [
+    (np.float32, None, 300_001, 256 << 10),
+    (np.float32, 1e-3, 300_001, 128 << 10),
+    (np.float64, None, 90_017, 512 << 10),
+    (np.float64, 1e-7, 90_017, 64 << 10),
+    (np.float32, 1e-2, 5_000, 64 << 20),     # one segment
+    (np.float32, None, 0, 1 << 20),          # empty array
+])
+def test_offload_reload(gpu, dt, eb, n, seg):
+    import torch
+    from sleek_amd import container, transfer
+    from sleek_amd.datagen import exact_field
+
+    say(f"transfer {np.dtype(dt).name} eb={eb} n={n} segment={seg}")
+    a = exact_field(n, 11, dt) if n else np.zeros(0, dtype=dt)
+    x = torch.from_numpy(a).cuda()
+    host = transfer.offload(x, eb, segment_bytes=seg)
+    assert host.is_pinned()
+    buf = host.numpy()
+    m = container.parse(buf)
+    assert m.total_bytes == a.nbytes and m.dtype == dt
+    pos = 0
+    for off, ln, raw in m.segments:
+        part = a.view(np.uint8)[pos: pos + raw].view(dt)
+        assert buf[off: off + ln].tobytes() == po.compress(part, eb).tobytes()
+        pos += raw
+    y = transfer.reload(host)
+    torch.cuda.synchronize()
+    if eb is None or not n:
+        want = a
+    else:   # segments decode independently: the oracle's decode, segment by segment
+        want = np.concatenate([po.decompress(buf[o: o + ln], dt, eb)[0] for o, ln, _ in m.segments])
+    assert y.cpu().numpy().tobytes() == want.tobytes()

@@ -110,3 +110,21 @@ def test_gpu_memcpy_lines(tmp_path):
     lines = run("GPUmemcpy", str(src))
     assert lines[7].startswith("DtD throughput:") and float(lines[7].split()[-2]) > 0   # compress_decompress_inputs_single.py:235
     assert lines[3] == f"original size: {4 << 20} bytes"
+
+
+def test_verify_harness_on_an_input_tree(tmp_path):
+    """The reference-style verify run over a small tree of SDRBench-like raw files
+    (tests/verify_inputs.py): every run bit-for-bit equal to the CPU codec."""
+    import verify_inputs
+
+    (tmp_path / "setA").mkdir()
+    field(np.float32, 40_000, specials=False).tofile(tmp_path / "setA" / "a.f32")
+    field(np.float32, 16_384 * 3, specials=False).tofile(tmp_path / "setA" / "b.f32")
+    (tmp_path / "setB").mkdir()
+    np.linspace(-1, 1, 5_001, dtype=np.float32).tofile(tmp_path / "setB" / "ramp.dat")
+    v, t = verify_inputs.main([str(tmp_path)])
+    assert (v, t) == (21, 21)
+    (tmp_path / "d").mkdir()
+    field(np.float64, 20_000, specials=False).tofile(tmp_path / "d" / "x.d64")
+    v, t = verify_inputs.main([str(tmp_path / "d"), "--precision", "double"])
+    assert (v, t) == (7, 7)

@@ -49,3 +49,17 @@ def test_offload_reload(gpu, dt, eb, n, seg):
     else:   # segments decode independently: the oracle's decode, segment by segment
         want = np.concatenate([po.decompress(buf[o: o + ln], dt, eb)[0] for o, ln, _ in m.segments])
     assert y.cpu().numpy().tobytes() == want.tobytes()
+
+
+def test_sdr_file_to_device(gpu, tmp_path):
+    """A raw input file reaches the device intact through the pinned slabs."""
+    import torch
+    from sleek_amd import sdr
+
+    say("sdr to_device")
+    a = np.random.default_rng(3).standard_normal(300_000).astype(np.float64)
+    p = tmp_path / "x.d64"
+    a.tofile(p)
+    x = sdr.to_device(str(p), slab_bytes=1 << 20)
+    torch.cuda.synchronize()
+    assert x.dtype == torch.float64 and x.cpu().numpy().tobytes() == a.tobytes()

@@ -549,7 +549,7 @@ def main():
 
     total_in = 2 * nbytes * len(ebs)
     out = {
-        "metric": "compress+decompress GB/s (512^3 f32, lossless + 3 error bounds)",
+        "metric": "compress+decompress GB/s & % HBM peak; batched vs plain; serve p50/p99",
         "value": round(total_in / (t_total / args.steps) / 1e9, 2),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -562,7 +562,9 @@ def main():
         "dtype": "u32",
         "data": "synthetic",
         "config": {"workload": "512^3 float32 sinusoid+ramp field with 0.01% specials; per step compress and "
-                               "decompress in lossless and eb={1e-2,1e-3,1e-4}*range modes",
+                               "decompress in lossless and eb={1e-2,1e-3,1e-4}*range modes; value = total "
+                               "input bytes of the 8 calls / step time (GB/s); % of HBM peak in roofline and "
+                               "per_mode; batched vs plain in batched; serve p50/p99 in serve",
                    "shape": [int(round(512 * args.scale))] * 3, "bytes": nbytes},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": traffic,

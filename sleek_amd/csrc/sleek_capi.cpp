@@ -143,9 +143,14 @@ int decompress_impl(int elem_bytes, bool lossy, int thr_e, unsigned long long of
   const long long ws_chunks = ws_bytes / 8 - 8;
   const long long cap = capacity >= 0 ? capacity : (ws_chunks - 8) * SLEEK_CHUNK_BYTES;
   if (cap < 0) return SLEEK_EWORKSPACE;
-  // only the status word needs clearing: the offsets are written
-  hipError_t e = sleek::zero_launch((uint64_t*)ws, 8, stream);
-  if (e != hipSuccess) return hip_fail(e, "workspace clear");
+  // only the status word needs clearing (the offsets are written); the
+  // scan decoder's offset kernel clears it itself
+  hipError_t e = hipSuccess;
+  const int variant = g_decode_variant;   // one value for the clear and the launch
+  if (variant != sleek::kDecodeScan) {
+    e = sleek::zero_launch((uint64_t*)ws, 8, stream);
+    if (e != hipSuccess) return hip_fail(e, "workspace clear");
+  }
   sleek::DecodeArgs a;
   a.in = d_encoded;
   a.inlen = enclen;
@@ -156,7 +161,7 @@ int decompress_impl(int elem_bytes, bool lossy, int thr_e, unsigned long long of
   a.thr_e = thr_e;
   a.offs = offs;
   const int grid = grid_for(blocks, elem_bytes, lossy, true, chunks_of(cap));
-  e = sleek::decode_launch(elem_bytes, lossy, a, grid, g_decode_variant, stream);
+  e = sleek::decode_launch(elem_bytes, lossy, a, grid, variant, stream);
   if (e != hipSuccess) return hip_fail(e, "decode launch");
   return SLEEK_OK;
 }

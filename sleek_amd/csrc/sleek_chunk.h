@@ -583,6 +583,10 @@ constexpr int kScanPer = 32;
 __device__ __forceinline__ void scan_offsets(const DecodeArgs& a, long long* wsum)
 {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  // this launch opens the call: it clears the status word (the later
+  // kernels of the call only OR bits into it), so the call needs no
+  // separate clearing launch
+  if (t == 0) flag_store(a.ws + kWsStatus, 0);
   const long long hdr = uni(*(const long long*)a.in);
   long long nchunks = 0;
   uint64_t why = 0;

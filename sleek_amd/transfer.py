@@ -18,7 +18,7 @@ import torch
 
 from . import _lib, codec, container
 
-DEFAULT_SEGMENT = 64 << 20
+DEFAULT_SEGMENT = 128 << 20
 
 
 def _segments(nbytes, elem, segment_bytes):

@@ -10,6 +10,7 @@
 #include <stddef.h>
 #include <string.h>
 
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -41,13 +42,14 @@ long long ws_words_for(long long insize)
 
 long long ws_bytes_for(long long insize) { return ws_words_for(insize) * 8; }
 
-// tuning knobs (sleek_gpu_set_tuning); none changes the bytes produced
-int g_encode_variant = sleek::kEncodeDefault;
-int g_encode_grid = 0;
-int g_decode_grid = 0;
-int g_decode_variant = sleek::kDecodeScan;
-int g_lb_shift = 6;    // look-back groups of 64 chunks
-int g_lb_window = 4;   // initial look-back window, in groups
+// tuning knobs (sleek_gpu_set_tuning); none changes the bytes produced.
+// Atomic: calls on other host threads read them while a setter runs.
+std::atomic<int> g_encode_variant{sleek::kEncodeDefault};
+std::atomic<int> g_encode_grid{0};
+std::atomic<int> g_decode_grid{0};
+std::atomic<int> g_decode_variant{sleek::kDecodeScan};
+std::atomic<int> g_lb_shift{6};    // look-back groups of 64 chunks
+std::atomic<int> g_lb_window{4};   // initial look-back window, in groups
 
 // Per-device scratch of the synchronous entry points (grow-only).
 struct DeviceScratch {

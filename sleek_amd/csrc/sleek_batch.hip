@@ -160,10 +160,13 @@ __global__ __launch_bounds__(kScanThreads) void batch_offsets_kernel(const Batch
   scan_offsets(decode_args(items + blockIdx.x, ws), wsum);
 }
 
-// Decoder: one workgroup per tile; tiles beyond the stream's chunk count
-// (the plan sizes arrays by their capacity) are skipped.
-__global__ __launch_bounds__(kThreads) void batch_decode_kernel(const BatchItem* items, const uint32_t* tile_map,
-                                                                long long tiles, uint64_t* ws)
+// Decoder: workgroups stride over the tiles, one tile at a time; tiles
+// beyond the stream's chunk count (the plan sizes arrays by their capacity)
+// are skipped.  At most 64 VGPRs (eight waves per SIMD, as the single-array
+// decoder): the four element kinds' paths would otherwise take 81 and leave
+// five.
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void batch_decode_kernel(
+    const BatchItem* items, const uint32_t* tile_map, long long tiles, uint64_t* ws)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   for (long long t = blockIdx.x; t < tiles; t += gridDim.x) {
@@ -221,7 +224,10 @@ hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long lon
   if (n <= 0) return hipSuccess;
   uint32_t* const map = (uint32_t*)(ws + batch_map_word(n));
   const long long res = batch_resident_grid(decompress);
-  long long g = grid > 0 ? grid : res;
+  // the decoder has no cross-tile waits: an oversubscribed grid (32 tiles'
+  // worth of workgroups per resident slot, measured best of 1x..200x) lets
+  // the hardware overlap the tiles' descriptor and header latencies
+  long long g = grid > 0 ? grid : (decompress ? 32 * res : res);
   if (!decompress && res > 0 && g > res) g = res;   // the encoder's look-back: no more than the resident set
   if (g <= 0) g = 1024;
   if (g > tiles) g = tiles > 0 ? tiles : 1;

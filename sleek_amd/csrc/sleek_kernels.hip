@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
 // of the entries before it.  No cross-workgroup dependency, no offsets
 // table, one launch.
 template <typename W, bool LOSSY>
-__global__ __launch_bounds__(kThreads) void decode_fused_kernel(DecodeArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))) void decode_fused_kernel(DecodeArgs a)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   __shared__ long long red[2][kThreads / 64];

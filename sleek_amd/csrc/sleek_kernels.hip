@@ -405,7 +405,7 @@ __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 template <typename W, bool LOSSY>
 static hipError_t launch_encode(const EncodeArgs& a, int grid, int variant, hipStream_t stream)
 {
-  variant = resolve_variant(variant);
+  variant = resolve_variant(variant, (int)sizeof(W));
   if (variant == kEncodeWave) hipLaunchKernelGGL((encode_wave_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
   else if (variant == kEncodeWavePipe) hipLaunchKernelGGL((encode_wave2_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
   else if (variant == kEncodePair) hipLaunchKernelGGL((encode_pair_kernel<W, LOSSY>), dim3(grid), dim3(128), 0, stream, a);
@@ -443,7 +443,7 @@ hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int gr
 template <typename W, bool LOSSY>
 static const void* encode_fn(int variant)
 {
-  variant = resolve_variant(variant);
+  variant = resolve_variant(variant, (int)sizeof(W));
   if (variant == kEncodeWave) return (const void*)encode_wave_kernel<W, LOSSY>;
   if (variant == kEncodeWavePipe) return (const void*)encode_wave2_kernel<W, LOSSY>;
   if (variant == kEncodePair) return (const void*)encode_pair_kernel<W, LOSSY>;
@@ -464,7 +464,7 @@ int resident_grid(int elem_bytes, bool lossy, bool decode, int variant)
     fn = elem_bytes == 4 ? (lossy ? encode_fn<uint32_t, true>(variant) : encode_fn<uint32_t, false>(variant))
                          : (lossy ? encode_fn<uint64_t, true>(variant) : encode_fn<uint64_t, false>(variant));
   }
-  const int rv = resolve_variant(variant);
+  const int rv = resolve_variant(variant, elem_bytes);
   const int threads = decode ? kThreads : ((rv == kEncodeWave || rv == kEncodeWavePipe) ? 64 : (rv == kEncodePair ? 128 : kThreads));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);

@@ -126,8 +126,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     enc_pack(ec, cur, sc, stage);
     ChunkShape sn;
     if (more) {
+      __builtin_amdgcn_s_setprio(2);   // others' look-backs wait on this size
       sn = enc_shape(en, nxt);
       publish_chunk(en.a, en.st, en.c, en.nchunks, sn.csize);
+      __builtin_amdgcn_s_setprio(0);
     }
     enc_finish(ec, sc.csize, stage);
     if (!more) break;

@@ -48,8 +48,8 @@ std::atomic<int> g_encode_variant{sleek::kEncodeDefault};
 std::atomic<int> g_encode_grid{0};
 std::atomic<int> g_decode_grid{0};
 std::atomic<int> g_decode_variant{sleek::kDecodeScan};
-std::atomic<int> g_lb_shift{6};    // look-back groups of 64 chunks
-std::atomic<int> g_lb_window{4};   // initial look-back window, in groups
+std::atomic<int> g_lb_shift{5};    // look-back groups of 32 chunks
+std::atomic<int> g_lb_window{8};   // initial look-back window, in groups
 
 // Per-device scratch of the synchronous entry points (grow-only).
 struct DeviceScratch {
@@ -352,7 +352,7 @@ int sleek_gpu_set_decoder(int variant)
 int sleek_gpu_set_lookback(int group_chunks, int window_groups)
 {
   if (group_chunks < 0 || window_groups < 0 || window_groups > 64) return SLEEK_EINVAL;
-  int shift = sleek::kGroupShiftMax;
+  int shift = 5;   // the default: 32 chunks
   if (group_chunks > 0) {
     for (shift = sleek::kGroupShiftMin; shift <= sleek::kGroupShiftMax && (1 << shift) != group_chunks; shift++) {
     }
@@ -360,7 +360,7 @@ int sleek_gpu_set_lookback(int group_chunks, int window_groups)
   }
   std::lock_guard<std::mutex> lk(g_mu);
   g_lb_shift = shift;
-  g_lb_window = window_groups > 0 ? window_groups : 4;
+  g_lb_window = window_groups > 0 ? window_groups : 8;
   return SLEEK_OK;
 }
 

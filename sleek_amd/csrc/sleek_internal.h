@@ -39,8 +39,8 @@ struct DecodeArgs {
 // register choice; kEncodeWave: one 64-thread wave per chunk; kEncodeWavePipe:
 // one wave per chunk, pipelined over the wave's chunks; kEncodePair: two
 // waves per chunk (128 threads), pipelined the same way.  kEncodeDefault
-// resolves to the fastest measured per element type: kEncodeWavePipe for
-// float32, kEncodePair for float64 (3-6 % faster there).  Output bytes never
+// resolves to the fastest measured, kEncodePair (4 waves per SIMD; 4-8 %
+// faster than kEncodeWavePipe on both element types).  Output bytes never
 // depend on the variant or on the grid size.
 constexpr int kEncodeDefault = 0;
 constexpr int kEncodePipelined = 1;
@@ -51,7 +51,8 @@ constexpr int kEncodePair = 5;
 constexpr int kEncodeVariants = 6;
 inline int resolve_variant(int v, int elem_bytes)
 {
-  return v != kEncodeDefault ? v : (elem_bytes == 8 ? kEncodePair : kEncodeWavePipe);
+  (void)elem_bytes;   // the same choice for both element types
+  return v != kEncodeDefault ? v : kEncodePair;
 }
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);

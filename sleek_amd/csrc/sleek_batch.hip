@@ -61,6 +61,7 @@ __device__ __forceinline__ EncTile enc_tile(const BatchItem* __restrict__ items,
   e.a.offs = it->offs;
   e.a.lb_shift = lb_shift;
   e.a.lb_window = lb_window;
+  e.a.tickets = nullptr;
   e.nchunks = (e.a.insize + kChunk - 1) / kChunk;
   e.c = t - it->tile0;
   e.kind = item_kind(it);

@@ -32,7 +32,7 @@ long long chunks_of(long long bytes) { return (bytes + SLEEK_CHUNK_BYTES - 1) / 
 // workspace: 8 header words (word 1: status), then per chunk one word, and
 // per look-back group (8 chunks at the smallest setting) two words (look-back
 // state when encoding; chunk offsets and their total when decoding); 8 spare
-// words, an even total
+// words, an even total.  A batch item's workspace is this much.
 long long ws_words_for(long long insize)
 {
   const long long ch = chunks_of(insize);
@@ -40,7 +40,8 @@ long long ws_words_for(long long insize)
   return sleek::even_words(16 + ch + 2 * ((ch + gmin - 1) / gmin));
 }
 
-long long ws_bytes_for(long long insize) { return ws_words_for(insize) * 8; }
+// a single-array call's workspace: the above, then the chunk-ticket counters
+long long ws_bytes_for(long long insize) { return (ws_words_for(insize) + sleek::kTicketWords) * 8; }
 
 // tuning knobs (sleek_gpu_set_tuning); none changes the bytes produced.
 // Atomic: calls on other host threads read them while a setter runs.
@@ -125,6 +126,7 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   a.offs = offs;
   a.lb_shift = g_lb_shift;
   a.lb_window = g_lb_window;
+  a.tickets = (unsigned int*)((uint64_t*)ws + ws_words_for(insize));
   const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
   e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_variant, stream);
   if (e != hipSuccess) return hip_fail(e, "encode launch");

@@ -418,6 +418,53 @@ __device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const Sca
 // wave clears and packs its own qwords of the staging buffer.
 constexpr int kHalfSub = kSub / 2;
 
+// ---- chunk tickets (encode_pair_kernel<.., TICKET>) ----
+// Tickets of set x: chunks grid + kTicketSets * t + x below nchunks.
+__device__ __forceinline__ uint32_t ticket_limit(long long nchunks, int x)
+{
+  const long long r = nchunks - (long long)gridDim.x - x;
+  return r > 0 ? (uint32_t)((r + kTicketSets - 1) / kTicketSets) : 0u;
+}
+
+// A ticket of the workgroup's own set, in lane 0 (other lanes: 0).  The
+// result is left in flight: it is read by ticket_chunk.
+__device__ __forceinline__ uint32_t ticket_request(uint32_t* tk)
+{
+  uint32_t t = 0;
+  if ((threadIdx.x & 63) == 0) t = atomicAdd(tk + (blockIdx.x & (kTicketSets - 1)) * kTicketStride, 1u);
+  return t;
+}
+
+// The chunk of a requested ticket (uniform), or -1 once every set is spent.
+// A ticket beyond its own set's limit falls back to the first other set whose
+// counter still has tickets (all seven read at once, one per lane).
+// Counters only grow, so a set seen spent stays spent and each ticket
+// below a limit goes to exactly one workgroup: every chunk is taken.  A set
+// whose ticket came back spent is not tried again, so the loop runs at most
+// seven times whatever the reads return.
+__device__ __noinline__ long long ticket_chunk(uint32_t* tk, uint32_t t, long long nchunks)
+{
+  const int lane = threadIdx.x & 63;
+  const int x = blockIdx.x & (kTicketSets - 1);
+  t = uni(t);
+  if (t < ticket_limit(nchunks, x)) return (long long)gridDim.x + (long long)kTicketSets * t + x;
+  uint32_t spent = 1u << x;
+  while (true) {
+    const int y = (x + 1 + lane) & (kTicketSets - 1);
+    bool open = false;
+    if (lane < kTicketSets - 1 && !((spent >> y) & 1u))
+      open = __hip_atomic_load(tk + y * kTicketStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ticket_limit(nchunks, y);
+    const uint64_t m = __ballot(open);
+    if (m == 0) return -1;
+    const int z = (x + 1 + __builtin_ctzll(m)) & (kTicketSets - 1);
+    uint32_t u = 0;
+    if (lane == 0) u = atomicAdd(tk + z * kTicketStride, 1u);
+    u = uni(u);
+    if (u < ticket_limit(nchunks, z)) return (long long)gridDim.x + (long long)kTicketSets * u + z;
+    spent |= 1u << z;
+  }
+}
+
 __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int h, uint2 raw[kHalfSub])
 {
   const int lane = threadIdx.x & 63;

@@ -20,7 +20,17 @@ struct EncodeArgs {
   unsigned long long offs;
   int lb_shift;           // look-back groups of 2^lb_shift chunks (3..6)
   int lb_window;          // initial look-back window in groups (1..64)
+  unsigned int* tickets;  // zeroed chunk counters of kEncodePairTicket (kTicketSets, kTicketStride apart)
 };
+
+// Chunk tickets (kEncodePairTicket): workgroups code chunk blockIdx.x first,
+// then take chunks from one of kTicketSets counters (set = blockIdx.x mod
+// kTicketSets): ticket t of set x is chunk grid + kTicketSets * t + x.  The
+// counters sit kTicketStride dwords (512 bytes) apart, after the look-back
+// words of the workspace.
+constexpr int kTicketSets = 8;
+constexpr int kTicketStride = 128;
+constexpr long long kTicketWords = kTicketSets * kTicketStride / 2;   // 8-byte workspace words
 
 struct DecodeArgs {
   const uint8_t* in;      // stream (8-byte aligned)
@@ -38,21 +48,25 @@ struct DecodeArgs {
 // staging buffers allow); kEncodeFree: the same kernel at the compiler's
 // register choice; kEncodeWave: one 64-thread wave per chunk; kEncodeWavePipe:
 // one wave per chunk, pipelined over the wave's chunks; kEncodePair: two
-// waves per chunk (128 threads), pipelined the same way.  kEncodeDefault
-// resolves to the fastest measured, kEncodePair (4 waves per SIMD; 4-8 %
-// faster than kEncodeWavePipe on both element types).  Output bytes never
-// depend on the variant or on the grid size.
+// waves per chunk (128 threads), pipelined the same way; kEncodePairTicket:
+// kEncodePair with chunks handed out in arrival order (chunk tickets).  kEncodeDefault
+// resolves to the fastest measured per element type: kEncodePairTicket for
+// float32 (2-4 % faster lossless than kEncodePair, even lossy), kEncodePair
+// for float64 (2 % faster than with tickets); both two-wave forms run 4 waves
+// per SIMD and beat kEncodeWavePipe by 4-8 %.  Output bytes never depend on
+// the variant or on the grid size.
 constexpr int kEncodeDefault = 0;
 constexpr int kEncodePipelined = 1;
 constexpr int kEncodeFree = 2;
 constexpr int kEncodeWave = 3;
 constexpr int kEncodeWavePipe = 4;
 constexpr int kEncodePair = 5;
-constexpr int kEncodeVariants = 6;
+constexpr int kEncodePairTicket = 6;
+constexpr int kEncodeVariants = 7;
 inline int resolve_variant(int v, int elem_bytes)
 {
-  (void)elem_bytes;   // the same choice for both element types
-  return v != kEncodeDefault ? v : kEncodePair;
+  if (v != kEncodeDefault) return v;
+  return elem_bytes == 4 ? kEncodePairTicket : kEncodePair;
 }
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);

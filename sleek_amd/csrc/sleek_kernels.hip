@@ -277,12 +277,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 //   code their half of n, post its bit lengths -> barrier -> n's shape;
 //   wave 0: publish n's size, look back for c -> barrier -> both store
 //   their lines of c -> barrier (staging buffer free).
-template <typename W, bool LOSSY>
+//
+// TICKET: after its first chunk (blockIdx.x) a workgroup takes the chunk
+// after next from the chunk tickets (EncodeArgs::tickets), requested at the
+// start of an iteration and read after the look-back, so the atomic's round
+// trip overlaps the iteration.  Chunks then start in arrival order, and a
+// chunk's predecessors have mostly published their sizes by its look-back,
+// however far individual workgroups drift apart.
+template <typename W, bool LOSSY, bool TICKET>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) void encode_pair_kernel(EncodeArgs a)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   __shared__ uint32_t xl[2][8];
   __shared__ unsigned long long xexcl;
+  __shared__ long long xnext;
   const int h = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
@@ -299,21 +307,28 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   if (c >= nchunks) return;
   uint2 cur[kHalfSub];
   uint32_t lh[4];
+  uint32_t tq = 0;
+  if (TICKET && h == 0) tq = ticket_request(a.tickets);
   load_half(a, c, h, cur);
   shape_half<W, LOSSY>(a, c, h, cur, lh);
   if (lane == 0) {
 #pragma unroll
     for (int i = 0; i < 4; i++) xl[0][4 * h + i] = lh[i];
   }
+  if (TICKET && h == 0) {
+    const long long m = ticket_chunk(a.tickets, tq, nchunks);
+    if (lane == 0) xnext = m;
+  }
   __syncthreads();
   ChunkShape sc = shape_of<W>(a, c, xl[0]);
   if (h == 0) publish_chunk(a, st, c, nchunks, sc.csize);
+  long long n = TICKET ? uni(xnext) : c + gridDim.x;
   int par = 1;
   while (true) {
-    const long long n = c + gridDim.x;
-    const bool more = n < nchunks;
+    const bool more = TICKET ? n >= 0 : n < nchunks;
     uint2 nxt[kHalfSub];
     if (more) load_half(a, n, h, nxt);
+    if (TICKET && h == 0 && more) tq = ticket_request(a.tickets);
     pack_half<W, LOSSY>(a, c, h, cur, sc, stage);
     __builtin_amdgcn_s_setprio(2);   // n's size is waited on by others: as in encode_wave2_kernel
     if (more) {
@@ -331,19 +346,24 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       if (more) publish_chunk(a, st, n, nchunks, sn.csize);
       __builtin_amdgcn_s_setprio(0);
       const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
+      long long m = -1;
+      if (TICKET && more) m = ticket_chunk(a.tickets, tq, nchunks);
       if (lane == 0) {
         if (c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)sc.csize);
         xexcl = excl;
+        if (TICKET) xnext = m;
         if (c == 0) *(long long*)a.out = a.insize;
         if (c == nchunks - 1) *a.outsize = data_base + (long long)excl + sc.csize;
       }
     }
-    __syncthreads();   // c's stream offset known
+    __syncthreads();   // c's stream offset known (and the chunk after n)
     const long long excl = (long long)uni((long long)xexcl);
+    const long long nn = TICKET ? uni(xnext) : n + gridDim.x;
     pair_store(a.out + data_base + excl, stage, sc.csize, h);
     __syncthreads();   // staging buffer read before it is packed again
     if (!more) break;
     c = n;
+    n = nn;
     sc = sn;
     par ^= 1;
 #pragma unroll
@@ -417,7 +437,8 @@ static hipError_t launch_encode(const EncodeArgs& a, int grid, int variant, hipS
   variant = resolve_variant(variant, (int)sizeof(W));
   if (variant == kEncodeWave) hipLaunchKernelGGL((encode_wave_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
   else if (variant == kEncodeWavePipe) hipLaunchKernelGGL((encode_wave2_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
-  else if (variant == kEncodePair) hipLaunchKernelGGL((encode_pair_kernel<W, LOSSY>), dim3(grid), dim3(128), 0, stream, a);
+  else if (variant == kEncodePair) hipLaunchKernelGGL((encode_pair_kernel<W, LOSSY, false>), dim3(grid), dim3(128), 0, stream, a);
+  else if (variant == kEncodePairTicket) hipLaunchKernelGGL((encode_pair_kernel<W, LOSSY, true>), dim3(grid), dim3(128), 0, stream, a);
   else if (variant == kEncodeFree) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 1>), dim3(grid), dim3(kThreads), 0, stream, a);
   else hipLaunchKernelGGL((encode_kernel<W, LOSSY, 4>), dim3(grid), dim3(kThreads), 0, stream, a);
   return hipGetLastError();
@@ -455,7 +476,8 @@ static const void* encode_fn(int variant)
   variant = resolve_variant(variant, (int)sizeof(W));
   if (variant == kEncodeWave) return (const void*)encode_wave_kernel<W, LOSSY>;
   if (variant == kEncodeWavePipe) return (const void*)encode_wave2_kernel<W, LOSSY>;
-  if (variant == kEncodePair) return (const void*)encode_pair_kernel<W, LOSSY>;
+  if (variant == kEncodePair) return (const void*)encode_pair_kernel<W, LOSSY, false>;
+  if (variant == kEncodePairTicket) return (const void*)encode_pair_kernel<W, LOSSY, true>;
   return variant == kEncodeFree ? (const void*)encode_kernel<W, LOSSY, 1> : (const void*)encode_kernel<W, LOSSY, 4>;
 }
 
@@ -474,7 +496,7 @@ int resident_grid(int elem_bytes, bool lossy, bool decode, int variant)
                          : (lossy ? encode_fn<uint64_t, true>(variant) : encode_fn<uint64_t, false>(variant));
   }
   const int rv = resolve_variant(variant, elem_bytes);
-  const int threads = decode ? kThreads : ((rv == kEncodeWave || rv == kEncodeWavePipe) ? 64 : (rv == kEncodePair ? 128 : kThreads));
+  const int threads = decode ? kThreads : ((rv == kEncodeWave || rv == kEncodeWavePipe) ? 64 : (rv == kEncodePair || rv == kEncodePairTicket ? 128 : kThreads));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);
 }

@@ -75,8 +75,9 @@ def test_barriers_in_single_outer_loop(asm):
         assert b[k] == [0, 1, 1, 1], (k, b[k])
     # the two-wave encoder: the first chunk's bit-length exchange before the loop,
     # then per chunk the exchange, the stream offset and the staging buffer's reuse
+    # (with and without chunk tickets)
     pair = [k for k in b if "encode_pair_kernel" in k]
-    assert len(pair) == 4
+    assert len(pair) == 8
     for k in pair:
         assert b[k] == [0, 1, 1, 1], (k, b[k])
     fused += pair

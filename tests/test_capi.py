@@ -75,7 +75,7 @@ def test_sizes_and_messages():
     for st in (0, -1, -2, -3, -4, -5, -6, -7):
         assert lib.sleek_gpu_strerror(st)
     assert lib.sleek_gpu_set_tuning(-1, 0, 0) == _lib.EINVAL
-    assert lib.sleek_gpu_set_tuning(6, 0, 0) == _lib.EINVAL
+    assert lib.sleek_gpu_set_tuning(7, 0, 0) == _lib.EINVAL
     assert lib.sleek_gpu_set_tuning(0, -1, 0) == _lib.EINVAL
     assert lib.sleek_gpu_set_tuning(0, 0, 0) == 0
     for g, w in ((12, 4), (4, 4), (128, 4), (64, 65), (-1, 0), (8, -1)):

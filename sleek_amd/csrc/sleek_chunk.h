@@ -427,42 +427,60 @@ __device__ __forceinline__ uint32_t ticket_limit(long long nchunks, int x)
 }
 
 // A ticket of the workgroup's own set, in lane 0 (other lanes: 0).  The
-// result is left in flight: it is read by ticket_chunk.
+// result is left in flight: it is read by ticket_chunk.  The address is made
+// opaque so that the compiler does not turn the lane's atomic into a
+// wave-wide one, whose result it would wait for at once.  Callers issue it
+// before loads they wait for early (memory operations complete in issue
+// order as far as those waits are concerned).
 __device__ __forceinline__ uint32_t ticket_request(uint32_t* tk)
 {
   uint32_t t = 0;
-  if ((threadIdx.x & 63) == 0) t = atomicAdd(tk + (blockIdx.x & (kTicketSets - 1)) * kTicketStride, 1u);
+  if ((threadIdx.x & 63) == 0) {
+    uintptr_t p = (uintptr_t)(tk + (blockIdx.x & (kTicketSets - 1)) * kTicketStride);
+    asm volatile("" : "+v"(p));
+    t = __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t*)p, 1u, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+  }
   return t;
 }
 
-// The chunk of a requested ticket (uniform), or -1 once every set is spent.
-// A ticket beyond its own set's limit falls back to the first other set whose
-// counter still has tickets (all seven read at once, one per lane).
-// Counters only grow, so a set seen spent stays spent and each ticket
-// below a limit goes to exactly one workgroup: every chunk is taken.  A set
-// whose ticket came back spent is not tried again, so the loop runs at most
-// seven times whatever the reads return.
-__device__ __noinline__ long long ticket_chunk(uint32_t* tk, uint32_t t, long long nchunks)
+// Once the workgroup's own set is spent: a ticket of the first other set
+// whose counter still has tickets (all seven read at once, one per lane), or
+// -1 when every set is spent.  Counters only grow, so a set seen spent stays
+// spent and each ticket below a limit goes to exactly one workgroup: every
+// chunk is taken.  A set whose ticket came back spent is not tried again, so
+// the loop runs at most seven times whatever the reads return.
+__device__ __noinline__ long long ticket_fallback(uint32_t* tk, long long nchunks)
 {
+  typedef __attribute__((address_space(1))) uint32_t gu32;
+  gu32* const g = (gu32*)tk;
   const int lane = threadIdx.x & 63;
   const int x = blockIdx.x & (kTicketSets - 1);
-  t = uni(t);
-  if (t < ticket_limit(nchunks, x)) return (long long)gridDim.x + (long long)kTicketSets * t + x;
   uint32_t spent = 1u << x;
   while (true) {
     const int y = (x + 1 + lane) & (kTicketSets - 1);
     bool open = false;
     if (lane < kTicketSets - 1 && !((spent >> y) & 1u))
-      open = __hip_atomic_load(tk + y * kTicketStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ticket_limit(nchunks, y);
+      open = __hip_atomic_load(g + y * kTicketStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ticket_limit(nchunks, y);
     const uint64_t m = __ballot(open);
     if (m == 0) return -1;
     const int z = (x + 1 + __builtin_ctzll(m)) & (kTicketSets - 1);
     uint32_t u = 0;
-    if (lane == 0) u = atomicAdd(tk + z * kTicketStride, 1u);
+    if (lane == 0) u = __hip_atomic_fetch_add(g + z * kTicketStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     u = uni(u);
     if (u < ticket_limit(nchunks, z)) return (long long)gridDim.x + (long long)kTicketSets * u + z;
     spent |= 1u << z;
   }
+}
+
+// The chunk of a ticket from ticket_request (uniform), or -1 once every set
+// is spent.
+__device__ __forceinline__ long long ticket_chunk(uint32_t* tk, uint32_t t, long long nchunks)
+{
+  const int x = blockIdx.x & (kTicketSets - 1);
+  t = uni(t);
+  if (t < ticket_limit(nchunks, x)) return (long long)gridDim.x + (long long)kTicketSets * t + x;
+  return ticket_fallback(tk, nchunks);
 }
 
 __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int h, uint2 raw[kHalfSub])
@@ -480,6 +498,7 @@ __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int 
     const uint2* const safe = (const uint2*)(a.in + base);
 #pragma unroll
     for (int s = 0; s < kHalfSub; s++) raw[s] = *(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
+    wait_vm_drained();   // see wait_vm_drained
   }
 }
 

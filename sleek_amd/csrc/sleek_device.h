@@ -392,6 +392,14 @@ __device__ __forceinline__ uint64_t read_field(const uint32_t* st, int p)
   return (uint64_t)__builtin_amdgcn_alignbit(w1, w0, sh) | ((uint64_t)__builtin_amdgcn_alignbit(w2, w1, sh) << 32);
 }
 
+// s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt left at their
+// maxima).  Ends a rarely taken branch that loads into the same registers as
+// the common path it rejoins (a partial last chunk): without it the compiler
+// cannot tell at the join that those loads are done, and the common path's
+// next loads into those registers would wait for every memory operation in
+// flight, stores of the previous chunk included.
+__device__ __forceinline__ void wait_vm_drained() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // ---- scan state ----
 __device__ __forceinline__ uint64_t flag_load(const uint64_t* p)
 {

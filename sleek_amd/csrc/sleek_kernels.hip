@@ -327,8 +327,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   while (true) {
     const bool more = TICKET ? n >= 0 : n < nchunks;
     uint2 nxt[kHalfSub];
-    if (more) load_half(a, n, h, nxt);
     if (TICKET && h == 0 && more) tq = ticket_request(a.tickets);
+    if (more) load_half(a, n, h, nxt);
     pack_half<W, LOSSY>(a, c, h, cur, sc, stage);
     __builtin_amdgcn_s_setprio(2);   // n's size is waited on by others: as in encode_wave2_kernel
     if (more) {

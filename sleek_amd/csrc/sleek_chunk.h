@@ -243,6 +243,7 @@ __device__ __forceinline__ void load_raw(const EncodeArgs& a, long long c, uint2
     const uint2* const safe = (const uint2*)(a.in + base);
 #pragma unroll
     for (int s = 0; s < kSub; s++) raw[s] = *(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
+    wait_vm_drained();
   }
 }
 
@@ -498,7 +499,7 @@ __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int 
     const uint2* const safe = (const uint2*)(a.in + base);
 #pragma unroll
     for (int s = 0; s < kHalfSub; s++) raw[s] = *(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
-    wait_vm_drained();   // see wait_vm_drained
+    wait_vm_drained();
   }
 }
 

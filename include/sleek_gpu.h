@@ -102,7 +102,7 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
                                long long capacity, long long* d_decsize, void* d_workspace,
                                long long workspace_bytes, int blocks, void* stream);
 /* Launch-shape knobs; no reference counterpart.  encode_variant: 0 =
- * default (currently 6 for float32, 5 for float64), 1 = pipelined 256-thread encoder at a
+ * default (currently 5), 1 = pipelined 256-thread encoder at a
  * 4-waves-per-SIMD register budget, 2 = the same at the compiler's register
  * choice, 3 = one wave per chunk, 4 = one wave per chunk pipelined over the
  * wave's chunks, 5 = two waves per chunk pipelined the same way, 6 = 5

@@ -27,7 +27,7 @@ import os
 DEFAULT = {"encode_variant": 0, "encode_grid_mult": 0.0, "lb_group": 32, "lb_window": 8, "decoder": 0,
            "decode_grid_mult": 0.0}
 
-ENCODE_SHAPES = [(0, 0.0), (5, 0.0), (6, 0.0), (4, 0.0), (5, 0.75), (1, 0.0), (3, 0.0)]
+ENCODE_SHAPES = [(0, 0.0), (6, 0.0), (4, 0.0), (5, 0.75), (1, 0.0), (3, 0.0)]
 MIN_GAIN = 0.01   # a candidate must beat the best so far by 1 % (timing noise is about that)
 LOOKBACK = [(16, 8), (32, 4), (32, 16), (64, 4), (64, 8)]
 DECODE_SHAPES = [(0, 1.0), (0, 2.0), (1, 0.0)]   # (decoder, grid multiple; 0.0 = automatic)

@@ -50,11 +50,10 @@ struct DecodeArgs {
 // one wave per chunk, pipelined over the wave's chunks; kEncodePair: two
 // waves per chunk (128 threads), pipelined the same way; kEncodePairTicket:
 // kEncodePair with chunks handed out in arrival order (chunk tickets).  kEncodeDefault
-// resolves to the fastest measured per element type: kEncodePairTicket for
-// float32 (2-4 % faster lossless than kEncodePair, even lossy), kEncodePair
-// for float64 (2 % faster than with tickets); both two-wave forms run 4 waves
-// per SIMD and beat kEncodeWavePipe by 4-8 %.  Output bytes never depend on
-// the variant or on the grid size.
+// resolves to kEncodePair for both element types: with tickets the two-wave
+// encoder measures the same within 1-2 % either way, depending on the
+// device; both two-wave forms run 4 waves per SIMD and beat kEncodeWavePipe
+// by 4-8 %.  Output bytes never depend on the variant or on the grid size.
 constexpr int kEncodeDefault = 0;
 constexpr int kEncodePipelined = 1;
 constexpr int kEncodeFree = 2;
@@ -65,8 +64,8 @@ constexpr int kEncodePairTicket = 6;
 constexpr int kEncodeVariants = 7;
 inline int resolve_variant(int v, int elem_bytes)
 {
-  if (v != kEncodeDefault) return v;
-  return elem_bytes == 4 ? kEncodePairTicket : kEncodePair;
+  (void)elem_bytes;   // the same choice for both element types
+  return v != kEncodeDefault ? v : kEncodePair;
 }
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);

@@ -178,8 +178,20 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         helping = helped = true;
       }
     } else {
+      // a group's word, or its completion count once every member has
+      // reported (the count's bytes are then the group's aggregate)
       const long long gj = gend - 1 - lane;
-      const uint64_t fg = (lane >= gwin) ? kAgg : ((gj >= 0) ? flag_load(s.group + gj) : kInc);  // before group 0: inclusive 0
+      uint64_t fg = kAgg;                      // lanes beyond the window: 0
+      if (lane < gwin) {
+        if (gj < 0) {
+          fg = kInc;                           // before group 0: inclusive 0
+        } else {
+          fg = flag_load(s.group + gj);
+          const uint64_t n = flag_load((const uint64_t*)(s.count + gj));
+          if ((fg >> 62) == 0 && (long long)(n >> 32) == group_members(s, gj, nchunks))
+            fg = (gj == 0 ? kInc : kAgg) | (n & 0xffffffffull);
+        }
+      }
       const uint64_t inc = __ballot((fg >> 62) == 2);
       const uint64_t rdy = __ballot((fg >> 62) != 0);
       if (inc) {
@@ -314,19 +326,19 @@ __device__ __forceinline__ ChunkShape shape_codes(const EncodeArgs& a, long long
 }
 
 // Publishes chunk c's size (lane 0): its look-back word, its size-table
-// entry and its group's counter; the chunk that completes its group
-// publishes the group's aggregate.
+// entry and its group's completion count.  The count's reply is not waited
+// for: look_back reads a complete count as the group's aggregate, so the
+// aggregate is visible as soon as the last member's add lands.
 __device__ __forceinline__ void publish_chunk(const EncodeArgs& a, const ScanState& st, long long c, long long nchunks,
                                               int csize)
 {
+  (void)nchunks;
   if ((threadIdx.x & 63) == 0) {
     flag_max(st.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
     ((uint16_t*)(a.out + 8))[c] = (uint16_t)csize;
-    const unsigned long long old = __hip_atomic_fetch_add(st.count + (c >> st.gshift), (1ull << 32) | (unsigned)csize,
-                                                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const long long g = c >> st.gshift;
-    if ((long long)(old >> 32) + 1 == group_members(st, g, nchunks))
-      flag_max(st.group + g, (g == 0 ? kInc : kAgg) | ((old & 0xffffffffull) + (unsigned)csize));
+    // no reply waited for: a complete count is read as the group's aggregate by look_back
+    __hip_atomic_fetch_add(st.count + (c >> st.gshift), (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

@@ -444,19 +444,6 @@ __device__ __forceinline__ long long group_members(const ScanState& s, long long
   return min(1ll << s.gshift, nchunks - (g << s.gshift));
 }
 
-// Publish chunk c's compressed size: its own aggregate word, and the group's
-// aggregate when c is the last of its group to report.  Only the chunk's own
-// workgroup calls this (each chunk counts once in its group's counter).
-__device__ __forceinline__ void publish_size(const ScanState& s, long long c, long long nchunks, uint32_t csize)
-{
-  flag_max(s.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
-  const long long g = c >> s.gshift;
-  const unsigned long long old = __hip_atomic_fetch_add(s.count + g, (1ull << 32) | csize, __ATOMIC_RELAXED,
-                                                        __HIP_MEMORY_SCOPE_AGENT);
-  if ((long long)(old >> 32) + 1 == group_members(s, g, nchunks))
-    flag_max(s.group + g, (g == 0 ? kInc : kAgg) | ((old & 0xffffffffull) + csize));
-}
-
 // Publish chunk c's inclusive offset (and its group's, for the group's last chunk).
 __device__ __forceinline__ void publish_inclusive(const ScanState& s, long long c, long long nchunks, uint64_t incl)
 {

@@ -29,6 +29,10 @@ DEFAULT = {"encode_variant": 0, "encode_grid_mult": 0.0, "lb_group": 32, "lb_win
 
 ENCODE_SHAPES = [(0, 0.0), (6, 0.0), (4, 0.0), (5, 0.75), (1, 0.0), (3, 0.0)]
 MIN_GAIN = 0.01   # a candidate must beat the best so far by 1 % (timing noise is about that)
+CONFIRM = 3       # a winner is timed again this often, alternately with the default, before it is kept
+# knob groups confirmed one by one: (keys, 0 = timed by the encoder / 1 = by the decoder)
+KNOB_GROUPS = ((("encode_variant", "encode_grid_mult"), 0), (("lb_group", "lb_window"), 0),
+               (("decoder", "decode_grid_mult"), 1))
 LOOKBACK = [(16, 8), (32, 4), (32, 16), (64, 4), (64, 8)]
 DECODE_SHAPES = [(0, 1.0), (0, 2.0), (1, 0.0)]   # (decoder, grid multiple; 0.0 = automatic)
 
@@ -106,6 +110,19 @@ def search(measure, encode_shapes=ENCODE_SHAPES, lookback=LOOKBACK, decode_shape
         _, d = run(cfg)
         if d < best_dec * (1 - MIN_GAIN):
             best, best_dec = cfg, d
+    # A single timing can win by noise: each knob group's winner must beat
+    # the group's default again, in CONFIRM rounds timed alternately with it
+    # (medians compared, the other groups as chosen), or the default stays.
+    for keys, side in KNOB_GROUPS:
+        if all(best[k] == DEFAULT[k] for k in keys):
+            continue
+        base = dict(best, **{k: DEFAULT[k] for k in keys})
+        tb, tc = [], []
+        for _ in range(CONFIRM):
+            tb.append(run(base)[side])
+            tc.append(run(best)[side])
+        if sorted(tc)[CONFIRM // 2] >= sorted(tb)[CONFIRM // 2] * (1 - MIN_GAIN):
+            best = base
     return best, trace
 
 

@@ -102,7 +102,7 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
                                long long capacity, long long* d_decsize, void* d_workspace,
                                long long workspace_bytes, int blocks, void* stream);
 /* Launch-shape knobs; no reference counterpart.  encode_variant: 0 =
- * default (currently 5), 1 = pipelined 256-thread encoder at a
+ * default (currently 6), 1 = pipelined 256-thread encoder at a
  * 4-waves-per-SIMD register budget, 2 = the same at the compiler's register
  * choice, 3 = one wave per chunk, 4 = one wave per chunk pipelined over the
  * wave's chunks, 5 = two waves per chunk pipelined the same way, 6 = 5
@@ -115,7 +115,8 @@ int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid);
  * reference watches a fixed window of 32 chunks, sleek_common_gpu.h:220-253):
  * group_chunks = chunks per look-back group, 8, 16, 32 or 64 (0 = default,
  * 32); window_groups = groups watched by the first group-level poll, 1..64
- * (0 = default, 8).  Speed only, never the bytes; applies to later calls. */
+ * (0 = default: 16 for encode_variant 6, 8 otherwise).  Speed only, never
+ * the bytes; applies to later calls. */
 int sleek_gpu_set_lookback(int group_chunks, int window_groups);
 /* Decoder shape (no reference counterpart): 0 = an offset-scan launch, then
  * one workgroup per chunk (the default); 1 = one persistent launch that

@@ -24,16 +24,16 @@ import json
 import math
 import os
 
-DEFAULT = {"encode_variant": 0, "encode_grid_mult": 0.0, "lb_group": 32, "lb_window": 8, "decoder": 0,
-           "decode_grid_mult": 0.0}
+DEFAULT = {"encode_variant": 0, "encode_grid_mult": 0.0, "lb_group": 32, "lb_window": 0, "decoder": 0,
+           "decode_grid_mult": 0.0}   # 0: the library's own choice (lb_window: by encoder)
 
-ENCODE_SHAPES = [(0, 0.0), (6, 0.0), (4, 0.0), (5, 0.75), (1, 0.0), (3, 0.0)]
+ENCODE_SHAPES = [(0, 0.0), (5, 0.0), (4, 0.0), (6, 0.75), (1, 0.0), (3, 0.0)]
 MIN_GAIN = 0.01   # a candidate must beat the best so far by 1 % (timing noise is about that)
 CONFIRM = 3       # a winner is timed again this often, alternately with the default, before it is kept
 # knob groups confirmed one by one: (keys, 0 = timed by the encoder / 1 = by the decoder)
 KNOB_GROUPS = ((("encode_variant", "encode_grid_mult"), 0), (("lb_group", "lb_window"), 0),
                (("decoder", "decode_grid_mult"), 1))
-LOOKBACK = [(16, 8), (32, 4), (32, 16), (64, 4), (64, 8)]
+LOOKBACK = [(32, 8), (32, 16), (16, 8), (64, 8), (64, 16), (32, 4)]
 DECODE_SHAPES = [(0, 1.0), (0, 2.0), (1, 0.0)]   # (decoder, grid multiple; 0.0 = automatic)
 
 

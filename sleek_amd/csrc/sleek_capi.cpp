@@ -50,7 +50,7 @@ std::atomic<int> g_encode_grid{0};
 std::atomic<int> g_decode_grid{0};
 std::atomic<int> g_decode_variant{sleek::kDecodeScan};
 std::atomic<int> g_lb_shift{5};    // look-back groups of 32 chunks
-std::atomic<int> g_lb_window{8};   // initial look-back window, in groups
+std::atomic<int> g_lb_window{0};   // initial look-back window, in groups (0: the encoder's default)
 
 // Per-device scratch of the synchronous entry points (grow-only).
 struct DeviceScratch {
@@ -125,7 +125,7 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   a.thr_e = thr_e;
   a.offs = offs;
   a.lb_shift = g_lb_shift;
-  a.lb_window = g_lb_window;
+  a.lb_window = g_lb_window > 0 ? (int)g_lb_window : sleek::default_window(sleek::resolve_variant(g_encode_variant, elem_bytes));
   a.tickets = (unsigned int*)((uint64_t*)ws + ws_words_for(insize));
   const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
   e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_variant, stream);
@@ -362,7 +362,7 @@ int sleek_gpu_set_lookback(int group_chunks, int window_groups)
   }
   std::lock_guard<std::mutex> lk(g_mu);
   g_lb_shift = shift;
-  g_lb_window = window_groups > 0 ? window_groups : 8;
+  g_lb_window = window_groups;   // 0: the encoder's default
   return SLEEK_OK;
 }
 
@@ -450,7 +450,8 @@ static int batch_async(bool decompress, const sleek_gpu_batch_item* d_items, int
   if (!aligned(ws, 16) || !aligned(d_items, 8)) return SLEEK_EALIGN;
   if (ws_bytes < (sleek::batch_map_word(n) + (tiles + 1) / 2) * 8) return SLEEK_EWORKSPACE;
   const hipError_t e = sleek::batch_launch(decompress, (const sleek::BatchItem*)d_items, n, tiles, (uint64_t*)ws, grid,
-                                           g_lb_shift, g_lb_window, (hipStream_t)stream);
+                                           g_lb_shift, g_lb_window > 0 ? (int)g_lb_window : sleek::kBatchWindow,
+                                           (hipStream_t)stream);
   return e == hipSuccess ? SLEEK_OK : hip_fail(e, "batch launch");
 }
 

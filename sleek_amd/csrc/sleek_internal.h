@@ -50,10 +50,11 @@ struct DecodeArgs {
 // one wave per chunk, pipelined over the wave's chunks; kEncodePair: two
 // waves per chunk (128 threads), pipelined the same way; kEncodePairTicket:
 // kEncodePair with chunks handed out in arrival order (chunk tickets).  kEncodeDefault
-// resolves to kEncodePair for both element types: with tickets the two-wave
-// encoder measures the same within 1-2 % either way, depending on the
-// device; both two-wave forms run 4 waves per SIMD and beat kEncodeWavePipe
-// by 4-8 %.  Output bytes never depend on the variant or on the grid size.
+// resolves to kEncodePairTicket for both element types: with its default
+// 16-group first look-back window it is 2-3 % faster than kEncodePair with
+// that one's default 8; both two-wave forms run 4 waves per SIMD and beat
+// kEncodeWavePipe by 4-8 %.  Output bytes never depend on the variant or on
+// the grid size.
 constexpr int kEncodeDefault = 0;
 constexpr int kEncodePipelined = 1;
 constexpr int kEncodeFree = 2;
@@ -65,8 +66,15 @@ constexpr int kEncodeVariants = 7;
 inline int resolve_variant(int v, int elem_bytes)
 {
   (void)elem_bytes;   // the same choice for both element types
-  return v != kEncodeDefault ? v : kEncodePair;
+  return v != kEncodeDefault ? v : kEncodePairTicket;
 }
+
+// The first look-back window (in groups) used when none is set: 16 for the
+// ticket encoder, whose chunks' predecessors publish in order, 8 for the
+// others (each measured best for its encoder); kBatchWindow for the batched
+// encoder.
+constexpr int kBatchWindow = 8;
+inline int default_window(int resolved_variant) { return resolved_variant == kEncodePairTicket ? 16 : 8; }
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);
 // Clears `words` 8-byte words with vector stores (no memset node when the

@@ -115,31 +115,28 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   load_raw(ec.a, ec.c, cur);
   ChunkShape sc = enc_shape(ec, cur);
   publish_chunk(ec.a, ec.st, ec.c, ec.nchunks, sc.csize);
-  while (true) {
-    const long long n = t + gridDim.x;
-    const bool more = n < tiles;
-    EncTile en;
+  // The last tile is finished after the loop, so the loop body has no
+  // branch on whether a next tile exists: with one, the compiler could not
+  // tell that the next tile's loads were consumed, and waited for every
+  // memory operation in flight (this tile's stores included) before the
+  // next iteration.
+  for (long long n = t + gridDim.x; n < tiles; n += gridDim.x) {
+    const EncTile en = enc_tile(items, tile_map, n, ws, lb_shift, lb_window);
     uint2 nxt[kSub];
-    if (more) {
-      en = enc_tile(items, tile_map, n, ws, lb_shift, lb_window);
-      load_raw(en.a, en.c, nxt);
-    }
+    load_raw(en.a, en.c, nxt);
     enc_pack(ec, cur, sc, stage);
-    ChunkShape sn;
-    if (more) {
-      __builtin_amdgcn_s_setprio(2);   // others' look-backs wait on this size
-      sn = enc_shape(en, nxt);
-      publish_chunk(en.a, en.st, en.c, en.nchunks, sn.csize);
-      __builtin_amdgcn_s_setprio(0);
-    }
+    __builtin_amdgcn_s_setprio(2);   // others' look-backs wait on this size
+    const ChunkShape sn = enc_shape(en, nxt);
+    publish_chunk(en.a, en.st, en.c, en.nchunks, sn.csize);
+    __builtin_amdgcn_s_setprio(0);
     enc_finish(ec, sc.csize, stage);
-    if (!more) break;
-    t = n;
     ec = en;
     sc = sn;
 #pragma unroll
     for (int s = 0; s < kSub; s++) cur[s] = nxt[s];
   }
+  enc_pack(ec, cur, sc, stage);
+  enc_finish(ec, sc.csize, stage);
 }
 
 __device__ __forceinline__ DecodeArgs decode_args(const BatchItem* it, uint64_t* ws)

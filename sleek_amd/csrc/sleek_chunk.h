@@ -245,7 +245,7 @@ __device__ __forceinline__ void load_raw(const EncodeArgs& a, long long c, uint2
   if (osize == kChunk) {
     // full chunk: one pointer, constant offsets, no masking
 #pragma unroll
-    for (int s = 0; s < kSub; s++) raw[s] = p[s * (kSubBytes / 8)];
+    for (int s = 0; s < kSub; s++) raw[s] = gload(p + s * (kSubBytes / 8));
   } else {
     // an opaque copy of the lane's valid byte count chooses the addresses:
     // the compiler cannot tie it to the masking, so it cannot turn the
@@ -254,7 +254,7 @@ __device__ __forceinline__ void load_raw(const EncodeArgs& a, long long c, uint2
     asm volatile("" : "+v"(rem));
     const uint2* const safe = (const uint2*)(a.in + base);
 #pragma unroll
-    for (int s = 0; s < kSub; s++) raw[s] = *(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
+    for (int s = 0; s < kSub; s++) raw[s] = gload(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
     wait_vm_drained();
   }
 }
@@ -335,7 +335,7 @@ __device__ __forceinline__ void publish_chunk(const EncodeArgs& a, const ScanSta
   (void)nchunks;
   if ((threadIdx.x & 63) == 0) {
     flag_max(st.chunk + c, (c == 0 ? kInc : kAgg) | (uint64_t)csize);
-    ((uint16_t*)(a.out + 8))[c] = (uint16_t)csize;
+    gstore((uint16_t*)(a.out + 8) + c, (uint16_t)csize);
     // no reply waited for: a complete count is read as the group's aggregate by look_back
     __hip_atomic_fetch_add(st.count + (c >> st.gshift), (1ull << 32) | (unsigned)csize, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
@@ -404,8 +404,8 @@ __device__ __forceinline__ void finish_chunk(const EncodeArgs& a, const ScanStat
   if (lane == 0 && c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)csize);
   wave_store_unaligned(a.out + data_base + (long long)excl, stage, csize);
   if (lane == 0) {
-    if (c == 0) *(long long*)a.out = a.insize;
-    if (c == nchunks - 1) *a.outsize = data_base + (long long)excl + csize;
+    if (c == 0) gstore((long long*)a.out, a.insize);
+    if (c == nchunks - 1) gstore(a.outsize, data_base + (long long)excl + csize);
   }
   __syncthreads();   // stage read by the stores before it is reused
 }
@@ -631,14 +631,14 @@ __device__ __forceinline__ void pair_store(uint8_t* dst, const uint32_t* lds, in
   const int nfull = (len - head) >> 4;
   const int tail = len - head - (nfull << 4);
   const uint8_t* lb = (const uint8_t*)lds;
-  if (h == 0 && lane < head) dst[lane] = lb[lane];
+  if (h == 0 && lane < head) gstore(dst + lane, lb[lane]);
   if (h == 1 && lane < tail) {
     const int i = head + (nfull << 4) + lane;
-    dst[i] = lb[i];
+    gstore(dst + i, lb[i]);
   }
   for (int j = h * 64 + lane; j < nfull; j += 128) {
     const int p = head + (j << 4);
-    *(uint4*)(dst + p) = lds_read16(lds, p);
+    gstore((uint4*)(dst + p), lds_read16(lds, p));
   }
 }
 

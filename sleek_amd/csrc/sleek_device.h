@@ -9,6 +9,41 @@
 
 namespace sleek {
 
+// Loads and stores through global-address-space pointers.  Pointers that
+// come from kernel arguments are proven global by the compiler; those read
+// from memory (a batch item's arrays) are not, and their accesses would
+// become flat instructions, which also count as LDS operations: each LDS
+// wait would then wait for the array loads and stores in flight as well.
+// The access is made with a plain scalar or vector type of the same size:
+// HIP's vector types are classes, whose copy goes through a generic `this`
+// and would make the access flat again.
+typedef unsigned int graw2 __attribute__((ext_vector_type(2)));
+typedef unsigned int graw4 __attribute__((ext_vector_type(4)));
+template <int N> struct GRaw;
+template <> struct GRaw<1> { typedef uint8_t T; };
+template <> struct GRaw<2> { typedef uint16_t T; };
+template <> struct GRaw<4> { typedef uint32_t T; };
+template <> struct GRaw<8> { typedef graw2 T; };
+template <> struct GRaw<16> { typedef graw4 T; };
+
+template <typename T>
+__device__ __forceinline__ T gload(const T* p)
+{
+  typedef typename GRaw<sizeof(T)>::T R;
+  const R r = *(const __attribute__((address_space(1))) R*)p;
+  T t;
+  __builtin_memcpy(&t, &r, sizeof(T));
+  return t;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(T* p, T v)
+{
+  typedef typename GRaw<sizeof(T)>::T R;
+  R r;
+  __builtin_memcpy(&r, &v, sizeof(T));
+  *(__attribute__((address_space(1))) R*)p = r;
+}
+
 // ---- format constants (sleek_common_gpu.h:54, :317-318 of the reference) ----
 constexpr int kChunk = 16384;            // bytes of input per chunk
 constexpr int kSub = 32;                 // subchunks per chunk
@@ -490,14 +525,14 @@ __device__ __forceinline__ void wave_store_unaligned(uint8_t* dst, const uint32_
   const int nfull = (len - head) >> 4;
   const int tail = len - head - (nfull << 4);
   const uint8_t* lb = (const uint8_t*)lds;
-  if (l < head) dst[l] = lb[l];
+  if (l < head) gstore(dst + l, lb[l]);
   if (l >= 16 && l < 16 + tail) {
     const int i = head + (nfull << 4) + (l - 16);
-    dst[i] = lb[i];
+    gstore(dst + i, lb[i]);
   }
   for (int j = l; j < nfull; j += 64) {
     const int p = head + (j << 4);
-    *(uint4*)(dst + p) = lds_read16(lds, p);
+    gstore((uint4*)(dst + p), lds_read16(lds, p));
   }
 }
 
@@ -513,14 +548,14 @@ __device__ __forceinline__ void store_unaligned(uint8_t* dst, const uint32_t* ld
   const int nfull = (len - head) >> 4;
   const int tail = len - head - (nfull << 4);
   const uint8_t* lb = (const uint8_t*)lds;
-  if (t < head) dst[t] = lb[t];
+  if (t < head) gstore(dst + t, lb[t]);
   if (t >= 64 && t < 64 + tail) {
     const int i = head + (nfull << 4) + (t - 64);
-    dst[i] = lb[i];
+    gstore(dst + i, lb[i]);
   }
   for (int j = t; j < nfull; j += kThreads) {
     const int p = head + (j << 4);
-    *(uint4*)(dst + p) = lds_read16(lds, p);
+    gstore((uint4*)(dst + p), lds_read16(lds, p));
   }
 }
 

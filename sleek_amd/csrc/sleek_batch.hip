@@ -87,17 +87,56 @@ __device__ __forceinline__ void enc_pack(const EncTile& e, const uint2 slot[kSub
   else pack_codes<uint64_t, true>(e.a, e.c, slot, sh, stage);
 }
 
-__device__ __forceinline__ void enc_finish(const EncTile& e, int csize, const uint32_t* stage)
+__device__ __forceinline__ uint64_t enc_look_back(const EncTile& e)
 {
   // the element type only matters to a look-back that sizes a missing chunk itself
-  uint64_t excl = 0;
-  if (e.c != 0) {
-    if (e.kind == 0) excl = look_back<uint32_t, false>(e.a, e.st, e.c, e.nchunks);
-    else if (e.kind == 1) excl = look_back<uint32_t, true>(e.a, e.st, e.c, e.nchunks);
-    else if (e.kind == 2) excl = look_back<uint64_t, false>(e.a, e.st, e.c, e.nchunks);
-    else excl = look_back<uint64_t, true>(e.a, e.st, e.c, e.nchunks);
+  if (e.c == 0) return 0;
+  if (e.kind == 0) return look_back<uint32_t, false>(e.a, e.st, e.c, e.nchunks);
+  if (e.kind == 1) return look_back<uint32_t, true>(e.a, e.st, e.c, e.nchunks);
+  if (e.kind == 2) return look_back<uint64_t, false>(e.a, e.st, e.c, e.nchunks);
+  return look_back<uint64_t, true>(e.a, e.st, e.c, e.nchunks);
+}
+
+__device__ __forceinline__ void enc_finish(const EncTile& e, int csize, const uint32_t* stage)
+{
+  finish_chunk(e.a, e.st, e.c, e.nchunks, csize, stage, enc_look_back(e));
+}
+
+// ---- two waves per tile (the single-array two-wave encoder's layout) ----
+__device__ __forceinline__ void enc_shape_half(const EncTile& e, int h, uint2 slot[kHalfSub], uint32_t lh[4])
+{
+  if (e.kind == 0) shape_half<uint32_t, false>(e.a, e.c, h, slot, lh);
+  else if (e.kind == 1) shape_half<uint32_t, true>(e.a, e.c, h, slot, lh);
+  else if (e.kind == 2) shape_half<uint64_t, false>(e.a, e.c, h, slot, lh);
+  else shape_half<uint64_t, true>(e.a, e.c, h, slot, lh);
+}
+
+__device__ __forceinline__ ChunkShape enc_shape_of(const EncTile& e, const uint32_t* xl)
+{
+  return e.kind >= 2 ? shape_of<uint64_t>(e.a, e.c, xl) : shape_of<uint32_t>(e.a, e.c, xl);
+}
+
+__device__ __forceinline__ void enc_pack_half(const EncTile& e, int h, const uint2 code[kHalfSub], const ChunkShape& sh,
+                                              uint32_t* stage)
+{
+  if (e.kind == 0) pack_half<uint32_t, false>(e.a, e.c, h, code, sh, stage);
+  else if (e.kind == 1) pack_half<uint32_t, true>(e.a, e.c, h, code, sh, stage);
+  else if (e.kind == 2) pack_half<uint64_t, false>(e.a, e.c, h, code, sh, stage);
+  else pack_half<uint64_t, true>(e.a, e.c, h, code, sh, stage);
+}
+
+// Wave 0, once the tile's payload is staged: the look-back, the inclusive
+// word, the stream header (first tile) and length (last tile); the tile's
+// stream offset goes to *xexcl.
+__device__ __forceinline__ void enc_settle(const EncTile& e, int csize, unsigned long long* xexcl)
+{
+  const uint64_t excl = enc_look_back(e);
+  if ((threadIdx.x & 63) == 0) {
+    if (e.c != 0) publish_inclusive(e.st, e.c, e.nchunks, excl + (uint64_t)csize);
+    *xexcl = excl;
+    if (e.c == 0) gstore((long long*)e.a.out, e.a.insize);
+    if (e.c == e.nchunks - 1) gstore(e.a.outsize, 8 + 2 * e.nchunks + (long long)excl + csize);
   }
-  finish_chunk(e.a, e.st, e.c, e.nchunks, csize, stage, excl);
 }
 
 // Encoder: one wave per tile, pipelined over the wave's tiles like the
@@ -137,6 +176,70 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   }
   enc_pack(ec, cur, sc, stage);
   enc_finish(ec, sc.csize, stage);
+}
+
+// Encoder, two waves per tile: wave h holds subchunks 16h .. 16h+15 of the
+// tile, as in the single-array two-wave encoder, and the next tile loads and
+// is coded while the current one is packed, looked back and stored.  The
+// last tile is finished after the loop, so the loop body has no branch on
+// whether a next tile exists.
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) void batch_encode_pair_kernel(
+    const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map, long long tiles, uint64_t* ws,
+    int lb_shift, int lb_window)
+{
+  __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
+  __shared__ uint32_t xl[2][8];
+  __shared__ unsigned long long xexcl;
+  const int h = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  long long t = blockIdx.x;
+  if (t >= tiles) return;
+  EncTile ec = enc_tile(items, tile_map, t, ws, lb_shift, lb_window);
+  uint2 cur[kHalfSub];
+  uint32_t lh[4];
+  load_half<true>(ec.a, ec.c, h, cur);
+  enc_shape_half(ec, h, cur, lh);
+  if (lane == 0) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) xl[0][4 * h + i] = lh[i];
+  }
+  __syncthreads();
+  ChunkShape sc = enc_shape_of(ec, xl[0]);
+  if (h == 0) publish_chunk(ec.a, ec.st, ec.c, ec.nchunks, sc.csize);
+  int par = 1;
+  for (long long n = t + gridDim.x; n < tiles; n += gridDim.x) {
+    const EncTile en = enc_tile(items, tile_map, n, ws, lb_shift, lb_window);
+    uint2 nxt[kHalfSub];
+    load_half<true>(en.a, en.c, h, nxt);
+    enc_pack_half(ec, h, cur, sc, stage);
+    __builtin_amdgcn_s_setprio(2);   // others' look-backs wait on this size
+    enc_shape_half(en, h, nxt, lh);
+    if (lane == 0) {
+#pragma unroll
+      for (int i = 0; i < 4; i++) xl[par][4 * h + i] = lh[i];
+    }
+    if (h == 1) __builtin_amdgcn_s_setprio(0);
+    __syncthreads();   // both halves packed; the next tile's bit lengths posted
+    const ChunkShape sn = enc_shape_of(en, xl[par]);
+    if (h == 0) {
+      publish_chunk(en.a, en.st, en.c, en.nchunks, sn.csize);
+      __builtin_amdgcn_s_setprio(0);
+      enc_settle(ec, sc.csize, &xexcl);
+    }
+    __syncthreads();   // the tile's stream offset known
+    pair_store(ec.a.out + 8 + 2 * ec.nchunks + (long long)uni((long long)xexcl), stage, sc.csize, h);
+    __syncthreads();   // staging buffer read before it is packed again
+    ec = en;
+    sc = sn;
+    par ^= 1;
+#pragma unroll
+    for (int s = 0; s < kHalfSub; s++) cur[s] = nxt[s];
+  }
+  enc_pack_half(ec, h, cur, sc, stage);
+  __syncthreads();
+  if (h == 0) enc_settle(ec, sc.csize, &xexcl);
+  __syncthreads();
+  pair_store(ec.a.out + 8 + 2 * ec.nchunks + (long long)uni((long long)xexcl), stage, sc.csize, h);
 }
 
 __device__ __forceinline__ DecodeArgs decode_args(const BatchItem* it, uint64_t* ws)
@@ -208,22 +311,24 @@ hipError_t zero_launch(uint64_t* p, long long words, hipStream_t stream)
   return hipGetLastError();
 }
 
-int batch_resident_grid(bool decompress)
+int batch_resident_grid(bool decompress, bool pair)
 {
   int dev = 0, cus = 0, per = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
-  const void* fn = decompress ? (const void*)batch_decode_kernel : (const void*)batch_encode_kernel;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, decompress ? kThreads : 64, 0) != hipSuccess) return 0;
+  const void* fn = decompress ? (const void*)batch_decode_kernel
+                              : (pair ? (const void*)batch_encode_pair_kernel : (const void*)batch_encode_kernel);
+  const int threads = decompress ? kThreads : (pair ? 128 : 64);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);
 }
 
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, hipStream_t stream)
+                        int lb_shift, int lb_window, bool pair, hipStream_t stream)
 {
   if (n <= 0) return hipSuccess;
   uint32_t* const map = (uint32_t*)(ws + batch_map_word(n));
-  const long long res = batch_resident_grid(decompress);
+  const long long res = batch_resident_grid(decompress, pair);
   // the decoder has no cross-tile waits: an oversubscribed grid (32 tiles'
   // worth of workgroups per resident slot, measured best of 1x..200x) lets
   // the hardware overlap the tiles' descriptor and header latencies
@@ -236,6 +341,9 @@ hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long lon
     hipLaunchKernelGGL(batch_offsets_kernel, dim3(n), dim3(kScanThreads), 0, stream, items, ws);
     if (tiles > 0)
       hipLaunchKernelGGL(batch_decode_kernel, dim3((unsigned)g), dim3(kThreads), 0, stream, items, map, tiles, ws);
+  } else if (tiles > 0 && pair) {
+    hipLaunchKernelGGL(batch_encode_pair_kernel, dim3((unsigned)g), dim3(128), 0, stream, items, map, tiles, ws,
+                       lb_shift, lb_window);
   } else if (tiles > 0) {
     hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)g), dim3(64), 0, stream, items, map, tiles, ws, lb_shift,
                        lb_window);

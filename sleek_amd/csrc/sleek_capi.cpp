@@ -449,8 +449,11 @@ static int batch_async(bool decompress, const sleek_gpu_batch_item* d_items, int
   if (n == 0) return SLEEK_OK;
   if (!aligned(ws, 16) || !aligned(d_items, 8)) return SLEEK_EALIGN;
   if (ws_bytes < (sleek::batch_map_word(n) + (tiles + 1) / 2) * 8) return SLEEK_EWORKSPACE;
+  // the two-wave single-array encoders select the two-wave batched encoder
+  const int rv = sleek::resolve_variant(g_encode_variant, 4);
+  const bool pair = rv == sleek::kEncodePair || rv == sleek::kEncodePairTicket;
   const hipError_t e = sleek::batch_launch(decompress, (const sleek::BatchItem*)d_items, n, tiles, (uint64_t*)ws, grid,
-                                           g_lb_shift, g_lb_window > 0 ? (int)g_lb_window : sleek::kBatchWindow,
+                                           g_lb_shift, g_lb_window > 0 ? (int)g_lb_window : sleek::kBatchWindow, pair,
                                            (hipStream_t)stream);
   return e == hipSuccess ? SLEEK_OK : hip_fail(e, "batch launch");
 }

@@ -496,6 +496,9 @@ __device__ __forceinline__ long long ticket_chunk(uint32_t* tk, uint32_t t, long
   return ticket_fallback(tk, nchunks);
 }
 
+// GLOBAL: the array pointer was read from memory (a batch item): load
+// through global-address-space pointers (gload).
+template <bool GLOBAL = false>
 __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int h, uint2 raw[kHalfSub])
 {
   const int lane = threadIdx.x & 63;
@@ -504,13 +507,16 @@ __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int 
   const uint2* const p = (const uint2*)(a.in + base + h * kHalfSub * kSubBytes) + lane;
   if (osize == kChunk) {
 #pragma unroll
-    for (int s = 0; s < kHalfSub; s++) raw[s] = p[s * (kSubBytes / 8)];
+    for (int s = 0; s < kHalfSub; s++) raw[s] = GLOBAL ? gload(p + s * (kSubBytes / 8)) : p[s * (kSubBytes / 8)];
   } else {
     int rem = osize - h * kHalfSub * kSubBytes - lane * 8;   // opaque: see load_raw
     asm volatile("" : "+v"(rem));
     const uint2* const safe = (const uint2*)(a.in + base);
 #pragma unroll
-    for (int s = 0; s < kHalfSub; s++) raw[s] = *(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
+    for (int s = 0; s < kHalfSub; s++) {
+      const uint2* const q = rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe;
+      raw[s] = GLOBAL ? gload(q) : *q;
+    }
     wait_vm_drained();
   }
 }

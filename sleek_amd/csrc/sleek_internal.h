@@ -115,9 +115,10 @@ constexpr long long kBatchHead = 16;
 inline long long even_words(long long w) { return (w + 1) & ~1ll; }
 inline long long batch_map_word(int n) { return kBatchHead + even_words(n); }
 
+// pair: the two-wave batched encoder (else one wave per tile)
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, hipStream_t stream);
+                        int lb_shift, int lb_window, bool pair, hipStream_t stream);
 hipError_t batch_status_launch(const BatchItem* items, int n, uint64_t* ws, hipStream_t stream);
-int batch_resident_grid(bool decompress);
+int batch_resident_grid(bool decompress, bool pair);
 
 }  // namespace sleek

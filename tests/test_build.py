@@ -76,13 +76,18 @@ def test_barriers_in_single_outer_loop(asm):
     # the two-wave encoder: the first chunk's bit-length exchange before the loop,
     # then per chunk the exchange, the stream offset and the staging buffer's reuse
     # (with and without chunk tickets)
-    pair = [k for k in b if "encode_pair_kernel" in k]
+    pair = [k for k in b if "encode_pair_kernel" in k and "batch" not in k]
     assert len(pair) == 8
     for k in pair:
         assert b[k] == [0, 1, 1, 1], (k, b[k])
     fused += pair
-    # single-wave workgroups (one-wave encoder, batched encoder) need no barrier instruction
-    assert not [k for k in b if "wave" in k or "batch_encode" in k]
+    # the two-wave batched encoder: the same, and the last tile's payload and
+    # stream offset after the loop (all waves leave the loop together)
+    bpair = [k for k in b if "batch_encode_pair_kernel" in k]
+    assert len(bpair) == 1 and b[bpair[0]] == [0, 1, 1, 1, 0, 0], b.get(bpair[0] if bpair else None)
+    fused += bpair
+    # single-wave workgroups (one-wave encoders, one-wave batched encoder) need no barrier instruction
+    assert not [k for k in b if "wave" in k or ("batch_encode" in k and "pair" not in k)]
     assert all(d == 1 for k, v in b.items() if k not in fused for d in v), b
 
 

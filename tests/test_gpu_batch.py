@@ -69,6 +69,27 @@ def test_batch_matches_oracle(gpu, grid):
             assert dec[i].tobytes() == a.tobytes()
 
 
+@pytest.mark.parametrize("variant", [4, 5])
+def test_batch_encoder_forms_match_oracle(gpu, variant):
+    """The batched encoder's one-wave form (encoder variant 4) and two-wave form
+    (variants 5 and 6, the default) write the oracle's streams, at the automatic
+    grid and at 5 workgroups."""
+    from sleek_amd import _lib
+
+    lib = _lib.load()
+    arrays, ebs = mixed_arrays(17, 25)
+    try:
+        for grid in (0, 5):
+            _lib.check(lib.sleek_gpu_set_tuning(variant, 0, 0), "tuning")
+            say(f"batch encoder variant={variant}, {len(arrays)} arrays, grid={grid}")
+            streams, dec, _, st_c, st_d = run_batch(arrays, ebs, grid)
+            assert not st_c.any() and not st_d.any()
+            for i, (a, eb) in enumerate(zip(arrays, ebs)):
+                assert streams[i].tobytes() == po.compress(a, eb).tobytes(), (variant, grid, i, a.size, a.dtype, eb)
+    finally:
+        lib.sleek_gpu_set_tuning(0, 0, 0)
+
+
 def test_batch_equals_single_array_calls(gpu):
     """A batch of larger arrays against the plain single-array entry points."""
     import torch

@@ -664,7 +664,9 @@ __device__ __forceinline__ bool header_ok(const DecodeArgs& a, long long hdr, lo
 // pass covers 32768 chunks (512 MiB of f32 input) and larger tables loop.
 // Also writes *decsize; a bad header is reported in the status word.
 constexpr int kScanThreads = 1024;
-constexpr int kScanPer = 32;
+constexpr int kScanRows = 8;                                           // rows of 256 entries per wave and round
+constexpr long long kScanBlock = 256 * kScanRows;                      // entries per wave and round
+constexpr long long kScanRound = kScanBlock * (kScanThreads / 64);     // entries per round
 __device__ __forceinline__ void scan_offsets(const DecodeArgs& a, long long* wsum)
 {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -683,57 +685,61 @@ __device__ __forceinline__ void scan_offsets(const DecodeArgs& a, long long* wsu
   const uint16_t* const tab = (const uint16_t*)(a.in + 8);
   long long* const offs = (long long*)(a.ws + kWsFlags);
   long long carry = 0;
-  for (long long base = 0; base < nchunks; base += (long long)kScanThreads * kScanPer) {
-    const long long i0 = base + (long long)t * kScanPer;
-    uint32_t v[kScanPer];
-    if (i0 + kScanPer <= nchunks) {
-      const unsigned long long* p = (const unsigned long long*)(tab + i0);   // 8-byte aligned
+  // Each wave takes kScanBlock consecutive entries per round, as kScanRows
+  // rows of 256: lane l holds entries 4l .. 4l+3 of every row (one 8-byte
+  // word), so the table loads and the offset stores are contiguous across
+  // the wave.  A wave scan per row gives each lane its entries' offsets.
+  for (long long base = 0; base < nchunks; base += kScanRound) {
+    const long long wb = base + (long long)wave * kScanBlock;
+    unsigned long long w[kScanRows];
+    uint32_t rs[kScanRows], incl[kScanRows];
 #pragma unroll
-      for (int k = 0; k < kScanPer / 4; k++) {
-        const unsigned long long w = p[k];
-        v[4 * k] = (uint32_t)(w & 0xffff);
-        v[4 * k + 1] = (uint32_t)((w >> 16) & 0xffff);
-        v[4 * k + 2] = (uint32_t)((w >> 32) & 0xffff);
-        v[4 * k + 3] = (uint32_t)(w >> 48);
+    for (int r = 0; r < kScanRows; r++) {
+      const long long i0 = wb + r * 256 + lane * 4;
+      if (i0 + 4 <= nchunks) {
+        w[r] = *(const unsigned long long*)(tab + i0);   // 8-byte aligned: i0 is a multiple of 4
+      } else {
+        w[r] = 0;
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (i0 + k < nchunks) w[r] |= (unsigned long long)tab[i0 + k] << (16 * k);
       }
-    } else {
-#pragma unroll
-      for (int k = 0; k < kScanPer; k++) v[k] = (i0 + k < nchunks) ? (uint32_t)tab[i0 + k] : 0u;
+      rs[r] = (uint32_t)(w[r] & 0xffff) + (uint32_t)((w[r] >> 16) & 0xffff) + (uint32_t)((w[r] >> 32) & 0xffff) +
+              (uint32_t)(w[r] >> 48);
     }
-    uint32_t local = 0;
+    uint32_t wtot = 0;
 #pragma unroll
-    for (int k = 0; k < kScanPer; k++) local += v[k];
-    uint32_t inc = local;   // inclusive wave scan of the per-thread sums
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t y = __shfl_up(inc, d, 64);
-      if (lane >= d) inc += y;
+    for (int r = 0; r < kScanRows; r++) {
+      incl[r] = wave_incl_scan(rs[r]);
+      wtot += (uint32_t)__builtin_amdgcn_readlane((int)incl[r], 63);
     }
-    if (lane == 63) wsum[wave] = inc;
+    if (lane == 0) wsum[wave] = wtot;
     __syncthreads();
     long long before = 0, tile = 0;
 #pragma unroll
-    for (int w = 0; w < kScanThreads / 64; w++) {
-      const long long x = wsum[w];
-      if (w < wave) before += x;
+    for (int v = 0; v < kScanThreads / 64; v++) {
+      const long long x = wsum[v];
+      if (v < wave) before += x;
       tile += x;
     }
-    long long run = carry + before + (long long)(inc - local);
-    if (i0 + kScanPer <= nchunks) {
-      longlong2* o = (longlong2*)(offs + i0);   // 16-byte aligned: i0 is a multiple of 32
+    long long row0 = carry + before;   // offset of the wave's first entry this round
 #pragma unroll
-      for (int k = 0; k < kScanPer / 2; k++) {
-        const long long r0 = run;
-        run += v[2 * k];
-        o[k] = make_longlong2(r0, run);
-        run += v[2 * k + 1];
-      }
-    } else {
+    for (int r = 0; r < kScanRows; r++) {
+      const long long i0 = wb + r * 256 + lane * 4;
+      long long o[4];
+      o[0] = row0 + (long long)(incl[r] - rs[r]);
 #pragma unroll
-      for (int k = 0; k < kScanPer; k++) {
-        if (i0 + k < nchunks) offs[i0 + k] = run;
-        run += v[k];
+      for (int k = 1; k < 4; k++) o[k] = o[k - 1] + (long long)((w[r] >> (16 * (k - 1))) & 0xffff);
+      if (i0 + 4 <= nchunks) {
+        longlong2* const q = (longlong2*)(offs + i0);   // 16-byte aligned: i0 is a multiple of 4
+        q[0] = make_longlong2(o[0], o[1]);
+        q[1] = make_longlong2(o[2], o[3]);
+      } else {
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+          if (i0 + k < nchunks) offs[i0 + k] = o[k];
       }
+      row0 += (long long)(uint32_t)__builtin_amdgcn_readlane((int)incl[r], 63);
     }
     carry += tile;
     __syncthreads();

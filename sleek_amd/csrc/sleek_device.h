@@ -381,6 +381,21 @@ __device__ __forceinline__ bool any_byte_above(uint32_t x, uint32_t n)
   return (((x + 0x01010101u * (127u - n)) | x) & 0x80808080u) != 0u;
 }
 
+// Inclusive sum over the wave's 64 lanes: Hillis-Steele within each row of
+// 16 lanes (DPP row shifts by 1, 2, 4, 8; lanes without a source add 0),
+// then row 0's total into row 1 and row 2's into row 3 (row_bcast:15), then
+// the first two rows' total into rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
+{
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);   // row_shr:1
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);   // row_shr:2
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);   // row_shr:4
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);   // row_shr:8
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);   // row_bcast:15, rows 1 and 3
+  x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);   // row_bcast:31, rows 2 and 3
+  return x;
+}
+
 // Sum of the four bytes of x.
 __device__ __forceinline__ uint32_t byte_sum(uint32_t x) { return __builtin_amdgcn_sad_u8(x, 0u, 0u); }
 

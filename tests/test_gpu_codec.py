@@ -227,6 +227,36 @@ def test_config_fields_scaled(gpu, config):
             assert maxabs_ok(a, d, eb)
 
 
+@pytest.mark.parametrize("chunks", [2048, 2049, 32768, 32769, 70001])
+def test_many_chunks_round_trip(gpu, chunks):
+    """Chunk counts around the offset scan's blocks (2,048 table entries per wave)
+    and rounds (32,768 per pass), up to 1.1 GiB: lossless round trips are
+    bit-exact, and both decoders (offset scan and fused) give the same bytes."""
+    import torch
+    from sleek_amd import _lib, codec
+
+    n = chunks * 4096 - 3
+    say(f"{chunks} chunks ({n} float32)")
+    g = torch.Generator(device="cuda").manual_seed(chunks)
+    x = (torch.randn(n, device="cuda", generator=g) * 0.01).cumsum(0)   # a compressible walk
+    lib = _lib.load()
+    try:
+        for eb in (None, 1e-3):
+            out, sz = codec.compress(x, eb)
+            outs = []
+            for dec in (0, 1):
+                _lib.check(lib.sleek_gpu_set_decoder(dec), "decoder")
+                d, dn = codec.decompress(out, x.dtype, eb, numel=n)
+                torch.cuda.synchronize()
+                assert int(dn.item()) == 4 * n
+                outs.append(d)
+            assert torch.equal(outs[0].view(torch.int32), outs[1].view(torch.int32)), eb
+            if eb is None:
+                assert torch.equal(outs[0].view(torch.int32), x.view(torch.int32))
+    finally:
+        lib.sleek_gpu_set_decoder(0)
+
+
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
 def test_empty_and_tiny(gpu, dt):
     say(f"empty/tiny {np.dtype(dt).name}")

@@ -115,7 +115,7 @@ int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid);
  * reference watches a fixed window of 32 chunks, sleek_common_gpu.h:220-253):
  * group_chunks = chunks per look-back group, 8, 16, 32 or 64 (0 = default,
  * 32); window_groups = groups watched by the first group-level poll, 1..64
- * (0 = default: 16 for encode_variant 6, 8 otherwise).  Speed only, never
+ * (0 = default: 64 for encode_variant 6, 8 otherwise).  Speed only, never
  * the bytes; applies to later calls. */
 int sleek_gpu_set_lookback(int group_chunks, int window_groups);
 /* Decoder shape (no reference counterpart): 0 = an offset-scan launch, then

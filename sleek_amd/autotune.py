@@ -33,7 +33,7 @@ CONFIRM = 3       # a winner is timed again this often, alternately with the def
 # knob groups confirmed one by one: (keys, 0 = timed by the encoder / 1 = by the decoder)
 KNOB_GROUPS = ((("encode_variant", "encode_grid_mult"), 0), (("lb_group", "lb_window"), 0),
                (("decoder", "decode_grid_mult"), 1))
-LOOKBACK = [(32, 8), (32, 16), (16, 8), (64, 8), (64, 16), (32, 4)]
+LOOKBACK = [(32, 8), (32, 16), (32, 32), (16, 8), (64, 8), (64, 16)]
 DECODE_SHAPES = [(0, 1.0), (0, 2.0), (1, 0.0)]   # (decoder, grid multiple; 0.0 = automatic)
 
 

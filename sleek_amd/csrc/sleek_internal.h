@@ -51,9 +51,9 @@ struct DecodeArgs {
 // waves per chunk (128 threads), pipelined the same way; kEncodePairTicket:
 // kEncodePair with chunks handed out in arrival order (chunk tickets).  kEncodeDefault
 // resolves to kEncodePairTicket for both element types: with its default
-// 16-group first look-back window it is 2-3 % faster than kEncodePair with
-// that one's default 8; both two-wave forms run 4 waves per SIMD and beat
-// kEncodeWavePipe by 4-8 %.  Output bytes never depend on the variant or on
+// 64-group first look-back window it is about 5 % faster than kEncodePair
+// with that one's default 8; both two-wave forms run 4 waves per SIMD and
+// beat kEncodeWavePipe by 4-8 %.  Output bytes never depend on the variant or on
 // the grid size.
 constexpr int kEncodeDefault = 0;
 constexpr int kEncodePipelined = 1;
@@ -69,12 +69,12 @@ inline int resolve_variant(int v, int elem_bytes)
   return v != kEncodeDefault ? v : kEncodePairTicket;
 }
 
-// The first look-back window (in groups) used when none is set: 16 for the
-// ticket encoder, whose chunks' predecessors publish in order, 8 for the
-// others (each measured best for its encoder); kBatchWindow for the batched
-// encoder.
+// The first look-back window (in groups) used when none is set: 64 for the
+// ticket encoder, whose nearest inclusive group word is about one iteration
+// (64 groups) back, so that one poll usually settles it; 8 for the others
+// (each measured best for its encoder); kBatchWindow for the batched encoder.
 constexpr int kBatchWindow = 8;
-inline int default_window(int resolved_variant) { return resolved_variant == kEncodePairTicket ? 16 : 8; }
+inline int default_window(int resolved_variant) { return resolved_variant == kEncodePairTicket ? 64 : 8; }
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);
 // Clears `words` 8-byte words with vector stores (no memset node when the

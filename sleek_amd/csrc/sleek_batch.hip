@@ -182,18 +182,24 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 // tile, as in the single-array two-wave encoder, and the next tile loads and
 // is coded while the current one is packed, looked back and stored.  The
 // last tile is finished after the loop, so the loop body has no branch on
-// whether a next tile exists.
+// whether a next tile exists.  TICKET: after its first tile (blockIdx.x) a
+// workgroup takes tiles from the tile tickets, as the single-array ticket
+// encoder takes chunks, so an array's tiles start in arrival order.
+template <bool TICKET>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) void batch_encode_pair_kernel(
     const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map, long long tiles, uint64_t* ws,
-    int lb_shift, int lb_window)
+    int lb_shift, int lb_window, uint32_t* tickets)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   __shared__ uint32_t xl[2][8];
   __shared__ unsigned long long xexcl;
+  __shared__ long long xnext;
   const int h = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   long long t = blockIdx.x;
   if (t >= tiles) return;
+  uint32_t tq = 0;
+  if (TICKET && h == 0) tq = ticket_request(tickets);
   EncTile ec = enc_tile(items, tile_map, t, ws, lb_shift, lb_window);
   uint2 cur[kHalfSub];
   uint32_t lh[4];
@@ -203,11 +209,17 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 #pragma unroll
     for (int i = 0; i < 4; i++) xl[0][4 * h + i] = lh[i];
   }
+  if (TICKET && h == 0) {
+    const long long m = ticket_chunk(tickets, tq, tiles);
+    if (lane == 0) xnext = m;
+  }
   __syncthreads();
   ChunkShape sc = enc_shape_of(ec, xl[0]);
   if (h == 0) publish_chunk(ec.a, ec.st, ec.c, ec.nchunks, sc.csize);
   int par = 1;
-  for (long long n = t + gridDim.x; n < tiles; n += gridDim.x) {
+  long long n = TICKET ? uni(xnext) : t + gridDim.x;
+  while (TICKET ? n >= 0 : n < tiles) {
+    if (TICKET && h == 0) tq = ticket_request(tickets);
     const EncTile en = enc_tile(items, tile_map, n, ws, lb_shift, lb_window);
     uint2 nxt[kHalfSub];
     load_half<true>(en.a, en.c, h, nxt);
@@ -225,8 +237,13 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       publish_chunk(en.a, en.st, en.c, en.nchunks, sn.csize);
       __builtin_amdgcn_s_setprio(0);
       enc_settle(ec, sc.csize, &xexcl);
+      if (TICKET) {
+        const long long m = ticket_chunk(tickets, tq, tiles);
+        if (lane == 0) xnext = m;
+      }
     }
-    __syncthreads();   // the tile's stream offset known
+    __syncthreads();   // the tile's stream offset known (and the tile after n)
+    const long long nn = TICKET ? uni(xnext) : n + gridDim.x;
     pair_store(ec.a.out + 8 + 2 * ec.nchunks + (long long)uni((long long)xexcl), stage, sc.csize, h);
     __syncthreads();   // staging buffer read before it is packed again
     ec = en;
@@ -234,6 +251,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     par ^= 1;
 #pragma unroll
     for (int s = 0; s < kHalfSub; s++) cur[s] = nxt[s];
+    n = nn;
   }
   enc_pack_half(ec, h, cur, sc, stage);
   __syncthreads();
@@ -317,14 +335,14 @@ int batch_resident_grid(bool decompress, bool pair)
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   const void* fn = decompress ? (const void*)batch_decode_kernel
-                              : (pair ? (const void*)batch_encode_pair_kernel : (const void*)batch_encode_kernel);
+                              : (pair ? (const void*)batch_encode_pair_kernel<true> : (const void*)batch_encode_kernel);
   const int threads = decompress ? kThreads : (pair ? 128 : 64);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);
 }
 
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, bool pair, hipStream_t stream)
+                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, hipStream_t stream)
 {
   if (n <= 0) return hipSuccess;
   uint32_t* const map = (uint32_t*)(ws + batch_map_word(n));
@@ -341,9 +359,14 @@ hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long lon
     hipLaunchKernelGGL(batch_offsets_kernel, dim3(n), dim3(kScanThreads), 0, stream, items, ws);
     if (tiles > 0)
       hipLaunchKernelGGL(batch_decode_kernel, dim3((unsigned)g), dim3(kThreads), 0, stream, items, map, tiles, ws);
+  } else if (tiles > 0 && pair && tickets) {
+    const hipError_t e = zero_launch((uint64_t*)tickets, kTicketWords, stream);   // a kernel: graph-capturable
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((batch_encode_pair_kernel<true>), dim3((unsigned)g), dim3(128), 0, stream, items, map, tiles,
+                       ws, lb_shift, lb_window, tickets);
   } else if (tiles > 0 && pair) {
-    hipLaunchKernelGGL(batch_encode_pair_kernel, dim3((unsigned)g), dim3(128), 0, stream, items, map, tiles, ws,
-                       lb_shift, lb_window);
+    hipLaunchKernelGGL((batch_encode_pair_kernel<false>), dim3((unsigned)g), dim3(128), 0, stream, items, map, tiles,
+                       ws, lb_shift, lb_window, (uint32_t*)nullptr);
   } else if (tiles > 0) {
     hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)g), dim3(64), 0, stream, items, map, tiles, ws, lb_shift,
                        lb_window);

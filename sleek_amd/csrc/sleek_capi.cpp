@@ -438,7 +438,7 @@ int sleek_gpu_batch_plan(sleek_gpu_batch_item* items, int n, int decompress, lon
     word += items[i].ws_words;
   }
   *tiles = tile;
-  *workspace_bytes = word * 8;
+  *workspace_bytes = (word + sleek::kTicketWords) * 8;   // then the tile-ticket counters
   return SLEEK_OK;
 }
 
@@ -449,12 +449,17 @@ static int batch_async(bool decompress, const sleek_gpu_batch_item* d_items, int
   if (n == 0) return SLEEK_OK;
   if (!aligned(ws, 16) || !aligned(d_items, 8)) return SLEEK_EALIGN;
   if (ws_bytes < (sleek::batch_map_word(n) + (tiles + 1) / 2) * 8) return SLEEK_EWORKSPACE;
-  // the two-wave single-array encoders select the two-wave batched encoder
+  // the two-wave single-array encoders select the two-wave batched encoder,
+  // the ticket one with tile tickets (counters in the workspace's last words)
   const int rv = sleek::resolve_variant(g_encode_variant, 4);
   const bool pair = rv == sleek::kEncodePair || rv == sleek::kEncodePairTicket;
+  if (rv == sleek::kEncodePairTicket && ws_bytes / 8 < sleek::batch_map_word(n) + (tiles + 1) / 2 + sleek::kTicketWords)
+    return SLEEK_EWORKSPACE;
+  uint32_t* const tickets =
+      rv == sleek::kEncodePairTicket ? (uint32_t*)((uint64_t*)ws + ws_bytes / 8 - sleek::kTicketWords) : nullptr;
   const hipError_t e = sleek::batch_launch(decompress, (const sleek::BatchItem*)d_items, n, tiles, (uint64_t*)ws, grid,
-                                           g_lb_shift, g_lb_window > 0 ? (int)g_lb_window : sleek::kBatchWindow, pair,
-                                           (hipStream_t)stream);
+                                           g_lb_shift, g_lb_window > 0 ? (int)g_lb_window : sleek::default_window(rv), pair,
+                                           tickets, (hipStream_t)stream);
   return e == hipSuccess ? SLEEK_OK : hip_fail(e, "batch launch");
 }
 

@@ -69,11 +69,11 @@ inline int resolve_variant(int v, int elem_bytes)
   return v != kEncodeDefault ? v : kEncodePairTicket;
 }
 
-// The first look-back window (in groups) used when none is set: 64 for the
-// ticket encoder, whose nearest inclusive group word is about one iteration
-// (64 groups) back, so that one poll usually settles it; 8 for the others
-// (each measured best for its encoder); kBatchWindow for the batched encoder.
-constexpr int kBatchWindow = 8;
+// The first look-back window (in groups) used when none is set, for the
+// single-array and the batched encoders alike: 64 with chunk (tile) tickets,
+// where the nearest inclusive group word is about one iteration (64 groups)
+// back, so that one poll usually settles it; 8 for the others (each measured
+// best for its encoder).
 inline int default_window(int resolved_variant) { return resolved_variant == kEncodePairTicket ? 64 : 8; }
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);
@@ -115,9 +115,11 @@ constexpr long long kBatchHead = 16;
 inline long long even_words(long long w) { return (w + 1) & ~1ll; }
 inline long long batch_map_word(int n) { return kBatchHead + even_words(n); }
 
-// pair: the two-wave batched encoder (else one wave per tile)
+// pair: the two-wave batched encoder (else one wave per tile); tickets: the
+// tile-ticket counters (kTicketWords words) for the two-wave encoder, or null
+// for tiles in a static order
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, bool pair, hipStream_t stream);
+                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, hipStream_t stream);
 hipError_t batch_status_launch(const BatchItem* items, int n, uint64_t* ws, hipStream_t stream);
 int batch_resident_grid(bool decompress, bool pair);
 

@@ -147,7 +147,7 @@ def test_batch_plan_layout(decompress):
         assert it["ws_word"] == word and it["ws_word"] % 2 == 0
         t += ch
         word += it["ws_words"]
-    assert tiles == t and wsb == word * 8
+    assert tiles == t and wsb == (word + 512) * 8   # then 4 KiB of tile-ticket counters
 
 
 def test_batch_plan_rejects_bad_items():

@@ -78,6 +78,20 @@ class Workspace:
         s = stream if stream is not None else torch.cuda.current_stream()
         return _lib.load().sleek_gpu_workspace_status(self.ptr(), s.cuda_stream)
 
+    def status_word(self):
+        """The device status word (int64 view, one element) that the last call on
+        this workspace left: its bits decode with status_of_word."""
+        return self.buf[8:16].view(torch.int64)
+
+
+def status_of_word(word):
+    """Status code of a workspace status word (see sleek_gpu_workspace_status)."""
+    if word & 2:
+        return _lib.ECAPACITY
+    if word & 1:
+        return _lib.ECORRUPT
+    return _lib.OK
+
 
 def _stream_handle(stream):
     s = stream if stream is not None else torch.cuda.current_stream()

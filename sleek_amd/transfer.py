@@ -11,7 +11,10 @@ single-array codec's stream of that segment, so a container's streams can
 also be decoded one by one with ``codec.decompress``.
 
 Two device stream buffers and workspaces (one per segment in flight) are
-the only device scratch; the host side is one pinned buffer.
+the only device scratch; the host side is one pinned buffer.  A workspace is
+reused every other segment, so each segment's status word and decoded length
+are copied to pinned host memory on the compute stream right after its
+decode, and every segment is checked before ``reload`` returns.
 """
 import numpy as np
 import torch
@@ -113,6 +116,9 @@ def reload(host, device=None, out=None):
     dbuf = [torch.empty(slen, dtype=torch.uint8, device=dev) for _ in range(2)]
     dsz = [torch.empty(1, dtype=torch.int64, device=dev) for _ in range(2)]
     ws = [codec.Workspace(raw_max, dev) for _ in range(2)]
+    # per segment: the workspace's status word and the decoded length, taken
+    # before the workspace and the length slot serve the segment after next
+    seen = torch.zeros((len(meta.segments), 2), dtype=torch.int64, pin_memory=True)
     comp, copy = torch.cuda.Stream(device=dev), torch.cuda.Stream(device=dev)
     comp.wait_stream(torch.cuda.current_stream(dev))
     copy.wait_stream(torch.cuda.current_stream(dev))
@@ -131,12 +137,18 @@ def reload(host, device=None, out=None):
             seg = flat[pos: pos + raw].view(tdt)
             codec.decompress(dbuf[b], meta.dtype, meta.eb, out=seg, decsize=dsz[b], workspace=ws[b], enclen=ln,
                              stream=comp)
+            seen[i, 0:1].copy_(ws[b].status_word(), non_blocking=True)
+            seen[i, 1:2].copy_(dsz[b], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(comp)
             done[b] = ev
         pos += raw
     comp.synchronize()
-    for w in ws[: len(meta.segments)]:
-        _lib.check(w.status(comp), "reload")
+    for i, (_, _, raw) in enumerate(meta.segments):
+        st, n = int(seen[i, 0]), int(seen[i, 1])
+        if st:
+            _lib.check(codec.status_of_word(st), f"reload: segment {i}")
+        if n != raw:
+            raise _lib.SleekError(_lib.ECORRUPT, f"reload: segment {i} decoded to {n} bytes, the container says {raw}")
     torch.cuda.current_stream(dev).wait_stream(comp)
     return out

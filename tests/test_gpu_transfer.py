@@ -63,3 +63,32 @@ def test_sdr_file_to_device(gpu, tmp_path):
     x = sdr.to_device(str(p), slab_bytes=1 << 20)
     torch.cuda.synchronize()
     assert x.dtype == torch.float64 and x.cpu().numpy().tobytes() == a.tobytes()
+
+
+@pytest.mark.parametrize("where", ["trailer", "header"])
+def test_reload_reports_a_damaged_early_segment(gpu, where):
+    """Damage in segment 0 of a multi-segment container is reported although the
+    two workspaces have served later segments since: each segment's status and
+    decoded length are checked."""
+    import torch
+    from sleek_amd import _lib, container, transfer
+    from sleek_amd.datagen import exact_field
+
+    say(f"transfer: damaged segment 0 ({where})")
+    a = exact_field(300_001, 12, np.float32)
+    x = torch.from_numpy(a).cuda()
+    host = transfer.offload(x, None, segment_bytes=256 << 10)
+    buf = host.numpy()
+    m = container.parse(buf)
+    assert len(m.segments) >= 4
+    off, ln, raw = m.segments[0]
+    if where == "trailer":
+        nch = (raw + 16383) // 16384
+        c0 = int(buf[off + 8: off + 10].view(np.uint16)[0])
+        assert c0 < 16384, "chunk 0 of segment 0 must be a packed chunk"
+        buf[off + 8 + 2 * nch + c0 - 2] ^= 0x5A   # chunk 0's int16 trailer
+    else:
+        buf[off: off + 8] = np.frombuffer(np.int64(raw - 4).tobytes(), np.uint8)   # a shorter decoded length
+    with pytest.raises(_lib.SleekError):
+        transfer.reload(host)
+    torch.cuda.synchronize()

@@ -48,6 +48,8 @@ SIGNATURES = [
     ("sleek_gpu_set_tuning", c_i, [c_i, c_i, c_i]),
     ("sleek_gpu_set_lookback", c_i, [c_i, c_i]),
     ("sleek_gpu_set_decoder", c_i, [c_i]),
+    ("sleek_gpu_set_help_after", c_i, [c_i]),
+    ("sleek_gpu_help_events", c_ll, []),
     ("sleek_gpu_workspace_status", c_i, [c_p, c_p]),
     ("sleek_gpu_batch_plan", c_i, [c_p, c_i, c_i, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)]),
     ("sleek_gpu_batch_compress_async", c_i, [c_p, c_i, c_ll, c_p, c_ll, c_i, c_p]),

@@ -46,8 +46,15 @@ struct EncTile {
   int kind;
 };
 
+// Look-back settings shared by every tile of a batched launch.
+struct LookCfg {
+  int lb_shift, lb_window;
+  unsigned long long* helps;
+  unsigned int help_ticks;
+};
+
 __device__ __forceinline__ EncTile enc_tile(const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map,
-                                            long long t, uint64_t* ws, int lb_shift, int lb_window)
+                                            long long t, uint64_t* ws, const LookCfg& lc)
 {
   const BatchItem* it = items + uni(tile_map[t]);
   EncTile e;
@@ -59,13 +66,15 @@ __device__ __forceinline__ EncTile enc_tile(const BatchItem* __restrict__ items,
   e.a.eb_e = it->eb_e;
   e.a.thr_e = it->thr_e;
   e.a.offs = it->offs;
-  e.a.lb_shift = lb_shift;
-  e.a.lb_window = lb_window;
+  e.a.lb_shift = lc.lb_shift;
+  e.a.lb_window = lc.lb_window;
   e.a.tickets = nullptr;
+  e.a.helps = lc.helps;
+  e.a.help_ticks = lc.help_ticks;
   e.nchunks = (e.a.insize + kChunk - 1) / kChunk;
   e.c = t - it->tile0;
   e.kind = item_kind(it);
-  e.st = scan_state(e.a.ws, e.nchunks, lb_shift);
+  e.st = scan_state(e.a.ws, e.nchunks, lc.lb_shift);
   return e;
 }
 
@@ -144,12 +153,12 @@ __device__ __forceinline__ void enc_settle(const EncTile& e, int csize, unsigned
 // is packed, its codes and size are made before the current look-back.
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void batch_encode_kernel(
     const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map, long long tiles, uint64_t* ws,
-    int lb_shift, int lb_window)
+    LookCfg lc)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   long long t = blockIdx.x;
   if (t >= tiles) return;
-  EncTile ec = enc_tile(items, tile_map, t, ws, lb_shift, lb_window);
+  EncTile ec = enc_tile(items, tile_map, t, ws, lc);
   uint2 cur[kSub];
   load_raw(ec.a, ec.c, cur);
   ChunkShape sc = enc_shape(ec, cur);
@@ -160,7 +169,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   // memory operation in flight (this tile's stores included) before the
   // next iteration.
   for (long long n = t + gridDim.x; n < tiles; n += gridDim.x) {
-    const EncTile en = enc_tile(items, tile_map, n, ws, lb_shift, lb_window);
+    const EncTile en = enc_tile(items, tile_map, n, ws, lc);
     uint2 nxt[kSub];
     load_raw(en.a, en.c, nxt);
     enc_pack(ec, cur, sc, stage);
@@ -182,25 +191,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 // tile, as in the single-array two-wave encoder, and the next tile loads and
 // is coded while the current one is packed, looked back and stored.  The
 // last tile is finished after the loop, so the loop body has no branch on
-// whether a next tile exists.  TICKET: after its first tile (blockIdx.x) a
-// workgroup takes tiles from the tile tickets, as the single-array ticket
-// encoder takes chunks, so an array's tiles start in arrival order.
+// whether a next tile exists.  TICKET: a workgroup takes every tile, its
+// first included, from the tile tickets, as the single-array ticket encoder
+// takes chunks, so an array's tiles start in arrival order and no tile waits
+// on a workgroup that has not started.
 template <bool TICKET>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) void batch_encode_pair_kernel(
     const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map, long long tiles, uint64_t* ws,
-    int lb_shift, int lb_window, uint32_t* tickets)
+    LookCfg lc, uint32_t* tickets)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   __shared__ uint32_t xl[2][8];
   __shared__ unsigned long long xexcl;
-  __shared__ long long xnext;
+  __shared__ long long xnext, xfirst;
   const int h = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   long long t = blockIdx.x;
+  if (TICKET) {
+    if (h == 0) {
+      const long long m = ticket_chunk(tickets, ticket_request(tickets), tiles);
+      if (lane == 0) xfirst = m;
+    }
+    __syncthreads();
+    t = uni(xfirst);
+    if (t < 0) return;   // uniform: every tile is taken
+  }
   if (t >= tiles) return;
   uint32_t tq = 0;
   if (TICKET && h == 0) tq = ticket_request(tickets);
-  EncTile ec = enc_tile(items, tile_map, t, ws, lb_shift, lb_window);
+  EncTile ec = enc_tile(items, tile_map, t, ws, lc);
   uint2 cur[kHalfSub];
   uint32_t lh[4];
   load_half<true>(ec.a, ec.c, h, cur);
@@ -220,7 +239,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   long long n = TICKET ? uni(xnext) : t + gridDim.x;
   while (TICKET ? n >= 0 : n < tiles) {
     if (TICKET && h == 0) tq = ticket_request(tickets);
-    const EncTile en = enc_tile(items, tile_map, n, ws, lb_shift, lb_window);
+    const EncTile en = enc_tile(items, tile_map, n, ws, lc);
     uint2 nxt[kHalfSub];
     load_half<true>(en.a, en.c, h, nxt);
     enc_pack_half(ec, h, cur, sc, stage);
@@ -342,8 +361,10 @@ int batch_resident_grid(bool decompress, bool pair)
 }
 
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, hipStream_t stream)
+                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, unsigned int help_ticks,
+                        hipStream_t stream)
 {
+  const LookCfg lc{lb_shift, lb_window, help_counter(), help_ticks};
   if (n <= 0) return hipSuccess;
   uint32_t* const map = (uint32_t*)(ws + batch_map_word(n));
   const long long res = batch_resident_grid(decompress, pair);
@@ -363,13 +384,12 @@ hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long lon
     const hipError_t e = zero_launch((uint64_t*)tickets, kTicketWords, stream);   // a kernel: graph-capturable
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((batch_encode_pair_kernel<true>), dim3((unsigned)g), dim3(128), 0, stream, items, map, tiles,
-                       ws, lb_shift, lb_window, tickets);
+                       ws, lc, tickets);
   } else if (tiles > 0 && pair) {
     hipLaunchKernelGGL((batch_encode_pair_kernel<false>), dim3((unsigned)g), dim3(128), 0, stream, items, map, tiles,
-                       ws, lb_shift, lb_window, (uint32_t*)nullptr);
+                       ws, lc, (uint32_t*)nullptr);
   } else if (tiles > 0) {
-    hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)g), dim3(64), 0, stream, items, map, tiles, ws, lb_shift,
-                       lb_window);
+    hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)g), dim3(64), 0, stream, items, map, tiles, ws, lc);
   }
   return hipGetLastError();
 }

@@ -51,6 +51,7 @@ std::atomic<int> g_decode_grid{0};
 std::atomic<int> g_decode_variant{sleek::kDecodeScan};
 std::atomic<int> g_lb_shift{5};    // look-back groups of 32 chunks
 std::atomic<int> g_lb_window{0};   // initial look-back window, in groups (0: the encoder's default)
+std::atomic<unsigned int> g_help_ticks{sleek::kHelpTicksDefault};   // look-back help threshold (100 MHz ticks)
 
 // Per-device scratch of the synchronous entry points (grow-only).
 struct DeviceScratch {
@@ -127,6 +128,8 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   a.lb_shift = g_lb_shift;
   a.lb_window = g_lb_window > 0 ? (int)g_lb_window : sleek::default_window(sleek::resolve_variant(g_encode_variant, elem_bytes));
   a.tickets = (unsigned int*)((uint64_t*)ws + ws_words_for(insize));
+  a.helps = sleek::help_counter();
+  a.help_ticks = g_help_ticks;
   const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
   e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_variant, stream);
   if (e != hipSuccess) return hip_fail(e, "encode launch");
@@ -334,7 +337,8 @@ int sleek_gpu_decompress_lossy_f64(int blocks, int eb_e, int thr_e, unsigned lon
 
 int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid)
 {
-  if (encode_variant < 0 || encode_variant >= sleek::kEncodeVariants || encode_grid < 0 || decode_grid < 0)
+  if (encode_variant < 0 || encode_variant >= sleek::kEncodeVariants || encode_variant == sleek::kEncodeRetired ||
+      encode_grid < 0 || decode_grid < 0)
     return SLEEK_EINVAL;
   std::lock_guard<std::mutex> lk(g_mu);
   g_encode_variant = encode_variant;
@@ -364,6 +368,25 @@ int sleek_gpu_set_lookback(int group_chunks, int window_groups)
   g_lb_shift = shift;
   g_lb_window = window_groups;   // 0: the encoder's default
   return SLEEK_OK;
+}
+
+int sleek_gpu_set_help_after(int microseconds)
+{
+  if (microseconds < 0 || microseconds > 10000000) return SLEEK_EINVAL;
+  g_help_ticks = (unsigned int)microseconds * 100u;   // s_memrealtime counts at 100 MHz
+  return SLEEK_OK;
+}
+
+long long sleek_gpu_help_events(void)
+{
+  unsigned long long* p = sleek::help_counter();
+  if (!p) return hip_fail(hipErrorInvalidSymbol, "help counter");
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
+  unsigned long long v = 0;
+  e = hipMemcpy(&v, p, 8, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpy help counter");
+  return (long long)v;
 }
 
 long long sleek_gpu_workspace_size(long long max_insize)
@@ -459,7 +482,7 @@ static int batch_async(bool decompress, const sleek_gpu_batch_item* d_items, int
       rv == sleek::kEncodePairTicket ? (uint32_t*)((uint64_t*)ws + ws_bytes / 8 - sleek::kTicketWords) : nullptr;
   const hipError_t e = sleek::batch_launch(decompress, (const sleek::BatchItem*)d_items, n, tiles, (uint64_t*)ws, grid,
                                            g_lb_shift, g_lb_window > 0 ? (int)g_lb_window : sleek::default_window(rv), pair,
-                                           tickets, (hipStream_t)stream);
+                                           tickets, g_help_ticks, (hipStream_t)stream);
   return e == hipSuccess ? SLEEK_OK : hip_fail(e, "batch launch");
 }
 

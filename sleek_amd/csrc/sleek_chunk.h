@@ -121,9 +121,12 @@ __device__ uint64_t wave_group_size(const EncodeArgs& a, const ScanState& s, lon
   return wave_sum(mine);
 }
 
-// Help threshold of a look-back: after this long (100 MHz real-time ticks)
-// without progress, the wave sizes the blocking chunk or group itself.
-constexpr uint64_t kHelpTicks = 10000;   // 100 us
+// One help event (a look-back sized a missing word itself) on the device's
+// counter, if there is one.
+__device__ __forceinline__ void count_help(unsigned long long* helps)
+{
+  if (helps) __hip_atomic_fetch_add(helps, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // Two-level decoupled look-back, run by one whole wave; returns the sum of
 // the compressed sizes of chunks 0..c-1 in every lane.  First the preceding
@@ -134,13 +137,15 @@ constexpr uint64_t kHelpTicks = 10000;   // 100 us
 // few words.  Groups hold 2^gshift chunks (8..64).
 // Polls back off exponentially: the words live at device scope, and many
 // waiting waves polling at full rate would compete with the data traffic.
-// Workgroups take chunks in launch order, so the words waited on are
-// normally published within a few round trips.  After kHelpTicks without
-// progress the waiter computes missing words from the input and publishes
-// them, one after another without further waits, until others' words arrive
-// again; so the look-back finishes even when a predecessor's workgroup is
-// not running (a co-running kernel holds the CUs, or the grid exceeds the
-// resident set).
+// Workgroups take chunks in launch or ticket order, so the words waited on
+// are normally published within a few round trips.  After a.help_ticks
+// without progress (100 us by default) the waiter computes missing
+// words from the input and publishes them, one after another without further
+// waits, until others' words arrive again; so the look-back finishes even
+// when a predecessor's workgroup is not running (an encoder with chunks in a
+// static order whose grid exceeds what is resident).  Each word sized so is
+// counted in *a.helps: the ticket encoders hand chunks only to running
+// workgroups and should never need it.
 template <typename W, bool LOSSY>
 __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long c, long long nchunks)
 {
@@ -168,12 +173,15 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         have_partial = true;
         progressed = true;
       }
-      if (!progressed && (helping || __builtin_amdgcn_s_memrealtime() - since > kHelpTicks)) {
+      if (!progressed && (helping || __builtin_amdgcn_s_memrealtime() - since > a.help_ticks)) {
         const uint64_t miss = __ballot(lane < q && (f >> 62) == 0);
         if (miss) {
           const long long jc = c - __ffsll((unsigned long long)miss);
           const uint32_t sz = wave_chunk_size<W, LOSSY>(a, jc);
-          if (lane == 0) flag_max(s.chunk + jc, kAgg | sz);
+          if (lane == 0) {
+            flag_max(s.chunk + jc, kAgg | sz);
+            count_help(a.helps);
+          }
         }
         helping = helped = true;
       }
@@ -204,12 +212,15 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         gwin = min(2 * gwin, 64);
         progressed = true;
       }
-      if (!progressed && (helping || __builtin_amdgcn_s_memrealtime() - since > kHelpTicks)) {
+      if (!progressed && (helping || __builtin_amdgcn_s_memrealtime() - since > a.help_ticks)) {
         const uint64_t miss = __ballot((fg >> 62) == 0);
         if (miss) {
           const long long gm = gend - __ffsll((unsigned long long)miss);
           const uint64_t sz = wave_group_size<W, LOSSY>(a, s, gm, nchunks);
-          if (lane == 0) flag_max(s.group + gm, (gm == 0 ? kInc : kAgg) | sz);
+          if (lane == 0) {
+            flag_max(s.group + gm, (gm == 0 ? kInc : kAgg) | sz);
+            count_help(a.helps);
+          }
         }
         helping = helped = true;
       }
@@ -410,19 +421,6 @@ __device__ __forceinline__ void finish_chunk(const EncodeArgs& a, const ScanStat
   __syncthreads();   // stage read by the stores before it is reused
 }
 
-// One chunk, start to end, by one wave (the batched encoder's tile).
-template <typename W, bool LOSSY>
-__device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const ScanState& st, long long c, long long nchunks,
-                                                  uint32_t* stage)
-{
-  uint2 slot[kSub];
-  load_raw(a, c, slot);
-  const ChunkShape sh = shape_codes<W, LOSSY>(a, c, slot);
-  publish_chunk(a, st, c, nchunks, sh.csize);
-  pack_codes<W, LOSSY>(a, c, slot, sh, stage);
-  finish_chunk(a, st, c, nchunks, sh.csize, stage, (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks));
-}
-
 // ---- two-wave chunk coder ----
 // Wave h of a 128-thread workgroup holds subchunks 16h .. 16h+15 of the
 // chunk (slice l of each in lane l, 32 VGPRs per chunk), half the
@@ -432,14 +430,21 @@ __device__ __forceinline__ void encode_chunk_wave(const EncodeArgs& a, const Sca
 constexpr int kHalfSub = kSub / 2;
 
 // ---- chunk tickets (encode_pair_kernel<.., TICKET>) ----
-// Tickets of set x: chunks grid + kTicketSets * t + x below nchunks.
-__device__ __forceinline__ uint32_t ticket_limit(long long nchunks, int x)
+// S = min(kTicketSets, grid) counters, so that every set has a workgroup of
+// its own (its home set, blockIdx.x mod S); ticket t of set x is chunk
+// S * t + x.  A workgroup takes every chunk, its first included, from the
+// counters, so a chunk is only ever held by a workgroup that runs.
+__device__ __forceinline__ int ticket_sets() { return gridDim.x < (unsigned)kTicketSets ? (int)gridDim.x : kTicketSets; }
+__device__ __forceinline__ int ticket_home() { return (int)(blockIdx.x % (unsigned)ticket_sets()); }
+
+// Tickets of set x: chunks S * t + x below nchunks.
+__device__ __forceinline__ uint32_t ticket_limit(long long nchunks, int x, int S)
 {
-  const long long r = nchunks - (long long)gridDim.x - x;
-  return r > 0 ? (uint32_t)((r + kTicketSets - 1) / kTicketSets) : 0u;
+  const long long r = nchunks - x;
+  return r > 0 ? (uint32_t)((r + S - 1) / S) : 0u;
 }
 
-// A ticket of the workgroup's own set, in lane 0 (other lanes: 0).  The
+// A ticket of the workgroup's home set, in lane 0 (other lanes: 0).  The
 // result is left in flight: it is read by ticket_chunk.  The address is made
 // opaque so that the compiler does not turn the lane's atomic into a
 // wave-wide one, whose result it would wait for at once.  Callers issue it
@@ -449,7 +454,7 @@ __device__ __forceinline__ uint32_t ticket_request(uint32_t* tk)
 {
   uint32_t t = 0;
   if ((threadIdx.x & 63) == 0) {
-    uintptr_t p = (uintptr_t)(tk + (blockIdx.x & (kTicketSets - 1)) * kTicketStride);
+    uintptr_t p = (uintptr_t)(tk + ticket_home() * kTicketStride);
     asm volatile("" : "+v"(p));
     t = __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t*)p, 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -457,31 +462,32 @@ __device__ __forceinline__ uint32_t ticket_request(uint32_t* tk)
   return t;
 }
 
-// Once the workgroup's own set is spent: a ticket of the first other set
-// whose counter still has tickets (all seven read at once, one per lane), or
+// Once the workgroup's home set is spent: a ticket of the first other set
+// whose counter still has tickets (all others read at once, one per lane), or
 // -1 when every set is spent.  Counters only grow, so a set seen spent stays
 // spent and each ticket below a limit goes to exactly one workgroup: every
 // chunk is taken.  A set whose ticket came back spent is not tried again, so
-// the loop runs at most seven times whatever the reads return.
+// the loop runs at most S - 1 times whatever the reads return.
 __device__ __noinline__ long long ticket_fallback(uint32_t* tk, long long nchunks)
 {
   typedef __attribute__((address_space(1))) uint32_t gu32;
   gu32* const g = (gu32*)tk;
   const int lane = threadIdx.x & 63;
-  const int x = blockIdx.x & (kTicketSets - 1);
+  const int S = ticket_sets();
+  const int x = ticket_home();
   uint32_t spent = 1u << x;
   while (true) {
-    const int y = (x + 1 + lane) & (kTicketSets - 1);
+    const int y = (x + 1 + lane) % S;
     bool open = false;
-    if (lane < kTicketSets - 1 && !((spent >> y) & 1u))
-      open = __hip_atomic_load(g + y * kTicketStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ticket_limit(nchunks, y);
+    if (lane < S - 1 && !((spent >> y) & 1u))
+      open = __hip_atomic_load(g + y * kTicketStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ticket_limit(nchunks, y, S);
     const uint64_t m = __ballot(open);
     if (m == 0) return -1;
-    const int z = (x + 1 + __builtin_ctzll(m)) & (kTicketSets - 1);
+    const int z = (x + 1 + __builtin_ctzll(m)) % S;
     uint32_t u = 0;
     if (lane == 0) u = __hip_atomic_fetch_add(g + z * kTicketStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     u = uni(u);
-    if (u < ticket_limit(nchunks, z)) return (long long)gridDim.x + (long long)kTicketSets * u + z;
+    if (u < ticket_limit(nchunks, z, S)) return (long long)S * u + z;
     spent |= 1u << z;
   }
 }
@@ -490,9 +496,10 @@ __device__ __noinline__ long long ticket_fallback(uint32_t* tk, long long nchunk
 // is spent.
 __device__ __forceinline__ long long ticket_chunk(uint32_t* tk, uint32_t t, long long nchunks)
 {
-  const int x = blockIdx.x & (kTicketSets - 1);
+  const int S = ticket_sets();
+  const int x = ticket_home();
   t = uni(t);
-  if (t < ticket_limit(nchunks, x)) return (long long)gridDim.x + (long long)kTicketSets * t + x;
+  if (t < ticket_limit(nchunks, x, S)) return (long long)S * t + x;
   return ticket_fallback(tk, nchunks);
 }
 

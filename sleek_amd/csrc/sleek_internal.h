@@ -21,11 +21,15 @@ struct EncodeArgs {
   int lb_shift;           // look-back groups of 2^lb_shift chunks (3..6)
   int lb_window;          // initial look-back window in groups (1..64)
   unsigned int* tickets;  // zeroed chunk counters of kEncodePairTicket (kTicketSets, kTicketStride apart)
+  unsigned long long* helps;   // device counter of look-back help events (help_counter())
+  unsigned int help_ticks;     // a look-back waiting this long (100 MHz ticks) sizes the missing words itself
 };
 
-// Chunk tickets (kEncodePairTicket): workgroups code chunk blockIdx.x first,
-// then take chunks from one of kTicketSets counters (set = blockIdx.x mod
-// kTicketSets): ticket t of set x is chunk grid + kTicketSets * t + x.  The
+// Chunk tickets (kEncodePairTicket): every chunk, a workgroup's first
+// included, comes from one of S = min(kTicketSets, grid) counters (home set
+// x = blockIdx.x mod S): ticket t of set x is chunk S * t + x.  A chunk is
+// therefore only ever held by a running workgroup, and every set has one, so
+// the look-back never waits on a workgroup that has not started.  The
 // counters sit kTicketStride dwords (512 bytes) apart, after the look-back
 // words of the workspace.
 constexpr int kTicketSets = 8;
@@ -46,19 +50,19 @@ struct DecodeArgs {
 // Encoder variants.  kEncodePipelined: 256-thread workgroups, three-stage
 // pipeline, register budget of 4 waves per SIMD (the most its two LDS
 // staging buffers allow); kEncodeFree: the same kernel at the compiler's
-// register choice; kEncodeWave: one 64-thread wave per chunk; kEncodeWavePipe:
-// one wave per chunk, pipelined over the wave's chunks; kEncodePair: two
-// waves per chunk (128 threads), pipelined the same way; kEncodePairTicket:
-// kEncodePair with chunks handed out in arrival order (chunk tickets).  kEncodeDefault
-// resolves to kEncodePairTicket for both element types: with its default
-// 64-group first look-back window it is about 5 % faster than kEncodePair
-// with that one's default 8; both two-wave forms run 4 waves per SIMD and
-// beat kEncodeWavePipe by 4-8 %.  Output bytes never depend on the variant or on
-// the grid size.
+// register choice; variant 3 (one unpipelined wave per chunk) is retired and
+// rejected; kEncodeWavePipe: one wave per chunk, pipelined over the wave's
+// chunks; kEncodePair: two waves per chunk (128 threads), pipelined the same
+// way; kEncodePairTicket: kEncodePair with chunks handed out in arrival
+// order (chunk tickets).  kEncodeDefault resolves to kEncodePairTicket for
+// both element types: with its default 64-group first look-back window it is
+// about 5 % faster than kEncodePair with that one's default 8; both two-wave
+// forms run 4 waves per SIMD and beat kEncodeWavePipe by 4-8 %.  Output bytes
+// never depend on the variant or on the grid size.
 constexpr int kEncodeDefault = 0;
 constexpr int kEncodePipelined = 1;
 constexpr int kEncodeFree = 2;
-constexpr int kEncodeWave = 3;
+constexpr int kEncodeRetired = 3;
 constexpr int kEncodeWavePipe = 4;
 constexpr int kEncodePair = 5;
 constexpr int kEncodePairTicket = 6;
@@ -75,6 +79,12 @@ inline int resolve_variant(int v, int elem_bytes)
 // back, so that one poll usually settles it; 8 for the others (each measured
 // best for its encoder).
 inline int default_window(int resolved_variant) { return resolved_variant == kEncodePairTicket ? 64 : 8; }
+
+// Default wait before a look-back sizes a missing predecessor itself: 100 us.
+constexpr unsigned int kHelpTicksDefault = 10000;
+// The device address of the calling device's help-event counter (null on a
+// HIP error); kernels add one per missing word they size themselves.
+unsigned long long* help_counter();
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);
 // Clears `words` 8-byte words with vector stores (no memset node when the
@@ -119,7 +129,8 @@ inline long long batch_map_word(int n) { return kBatchHead + even_words(n); }
 // tile-ticket counters (kTicketWords words) for the two-wave encoder, or null
 // for tiles in a static order
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, hipStream_t stream);
+                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, unsigned int help_ticks,
+                        hipStream_t stream);
 hipError_t batch_status_launch(const BatchItem* items, int n, uint64_t* ws, hipStream_t stream);
 int batch_resident_grid(bool decompress, bool pair);
 

@@ -14,9 +14,33 @@
 //   coalesced stores.
 // Reference counterparts: d_encode / d_decode, sleek_common_gpu.h:474-768
 // (lossless) and :989-1279 (lossy).
+#include <mutex>
+#include <vector>
+
 #include "sleek_chunk.h"
 
 namespace sleek {
+
+// Look-back help events on this device since the library was loaded (each
+// one a missing look-back word that a waiting wave sized itself).  Kernels
+// get its device address (help_counter) as an argument.
+__device__ unsigned long long help_events;
+
+unsigned long long* help_counter()
+{
+  static std::mutex mu;
+  static std::vector<unsigned long long*> addr;   // per device
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(help_events)) != hipSuccess) return nullptr;
+    addr[dev] = (unsigned long long*)p;
+  }
+  return addr[dev];
+}
 
 // Encoder: each workgroup walks its chunks (launch order, stride = grid) as
 // a three-stage pipeline, one chunk per stage per iteration:
@@ -194,27 +218,6 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
   }
 }
 
-// Encoder, one wave per chunk: a workgroup is a single wave that codes its
-// chunks (launch order, stride = grid) with encode_chunk_wave.  The private
-// 16 KiB LDS buffer allows nine such waves on a CU, registers eight; the
-// look-back and the load latencies of one wave are covered by the others'
-// arithmetic.
-template <typename W, bool LOSSY>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void encode_wave_kernel(EncodeArgs a)
-{
-  __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
-  const long long nchunks = (a.insize + kChunk - 1) / kChunk;
-  if (nchunks == 0) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-      *(long long*)a.out = 0;
-      *a.outsize = 8;
-    }
-    return;
-  }
-  const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
-  for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) encode_chunk_wave<W, LOSSY>(a, st, c, nchunks, stage);
-}
-
 // Encoder, one wave per chunk, pipelined across a wave's chunks (launch
 // order, stride = grid).  Iteration for chunk c, whose codes and size were
 // made in the previous iteration:
@@ -278,19 +281,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 //   wave 0: publish n's size, look back for c -> barrier -> both store
 //   their lines of c -> barrier (staging buffer free).
 //
-// TICKET: after its first chunk (blockIdx.x) a workgroup takes the chunk
-// after next from the chunk tickets (EncodeArgs::tickets), requested at the
-// start of an iteration and read after the look-back, so the atomic's round
-// trip overlaps the iteration.  Chunks then start in arrival order, and a
-// chunk's predecessors have mostly published their sizes by its look-back,
-// however far individual workgroups drift apart.
+// TICKET: a workgroup takes every chunk from the chunk tickets
+// (EncodeArgs::tickets): the first one before it starts, then the chunk
+// after next, requested at the start of an iteration and read after the
+// look-back, so the atomic's round trip overlaps the iteration.  Chunks then
+// start in arrival order, a chunk's predecessors have mostly published their
+// sizes by its look-back however far individual workgroups drift apart, and
+// no chunk waits on a workgroup that has not started.
 template <typename W, bool LOSSY, bool TICKET>
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) void encode_pair_kernel(EncodeArgs a)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   __shared__ uint32_t xl[2][8];
   __shared__ unsigned long long xexcl;
-  __shared__ long long xnext;
+  __shared__ long long xnext, xfirst;
   const int h = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
@@ -304,6 +308,15 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
   const long long data_base = 8 + 2 * nchunks;
   long long c = blockIdx.x;
+  if (TICKET) {
+    if (h == 0) {
+      const long long m = ticket_chunk(a.tickets, ticket_request(a.tickets), nchunks);
+      if (lane == 0) xfirst = m;
+    }
+    __syncthreads();
+    c = uni(xfirst);
+    if (c < 0) return;   // uniform: every chunk is taken
+  }
   if (c >= nchunks) return;
   uint2 cur[kHalfSub];
   uint32_t lh[4];
@@ -435,8 +448,7 @@ template <typename W, bool LOSSY>
 static hipError_t launch_encode(const EncodeArgs& a, int grid, int variant, hipStream_t stream)
 {
   variant = resolve_variant(variant, (int)sizeof(W));
-  if (variant == kEncodeWave) hipLaunchKernelGGL((encode_wave_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
-  else if (variant == kEncodeWavePipe) hipLaunchKernelGGL((encode_wave2_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
+  if (variant == kEncodeWavePipe) hipLaunchKernelGGL((encode_wave2_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
   else if (variant == kEncodePair) hipLaunchKernelGGL((encode_pair_kernel<W, LOSSY, false>), dim3(grid), dim3(128), 0, stream, a);
   else if (variant == kEncodePairTicket) hipLaunchKernelGGL((encode_pair_kernel<W, LOSSY, true>), dim3(grid), dim3(128), 0, stream, a);
   else if (variant == kEncodeFree) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 1>), dim3(grid), dim3(kThreads), 0, stream, a);
@@ -474,7 +486,6 @@ template <typename W, bool LOSSY>
 static const void* encode_fn(int variant)
 {
   variant = resolve_variant(variant, (int)sizeof(W));
-  if (variant == kEncodeWave) return (const void*)encode_wave_kernel<W, LOSSY>;
   if (variant == kEncodeWavePipe) return (const void*)encode_wave2_kernel<W, LOSSY>;
   if (variant == kEncodePair) return (const void*)encode_pair_kernel<W, LOSSY, false>;
   if (variant == kEncodePairTicket) return (const void*)encode_pair_kernel<W, LOSSY, true>;
@@ -496,7 +507,7 @@ int resident_grid(int elem_bytes, bool lossy, bool decode, int variant)
                          : (lossy ? encode_fn<uint64_t, true>(variant) : encode_fn<uint64_t, false>(variant));
   }
   const int rv = resolve_variant(variant, elem_bytes);
-  const int threads = decode ? kThreads : ((rv == kEncodeWave || rv == kEncodeWavePipe) ? 64 : (rv == kEncodePair || rv == kEncodePairTicket ? 128 : kThreads));
+  const int threads = decode ? kThreads : (rv == kEncodeWavePipe ? 64 : (rv == kEncodePair || rv == kEncodePairTicket ? 128 : kThreads));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);
 }

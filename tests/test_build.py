@@ -74,19 +74,21 @@ def test_barriers_in_single_outer_loop(asm):
     for k in fused:
         assert b[k] == [0, 1, 1, 1], (k, b[k])
     # the two-wave encoder: the first chunk's bit-length exchange before the loop,
-    # then per chunk the exchange, the stream offset and the staging buffer's reuse
-    # (with and without chunk tickets)
+    # then per chunk the exchange, the stream offset and the staging buffer's reuse;
+    # with chunk tickets, first the exchange of the first chunk's ticket
     pair = [k for k in b if "encode_pair_kernel" in k and "batch" not in k]
     assert len(pair) == 8
     for k in pair:
-        assert b[k] == [0, 1, 1, 1], (k, b[k])
+        ticket = k.endswith("Lb1EEEvNS_10EncodeArgsE")
+        assert b[k] == ([0] if ticket else []) + [0, 1, 1, 1], (k, b[k])
     fused += pair
     # the two-wave batched encoder: the same, and the last tile's payload and
     # stream offset after the loop (all waves leave the loop together)
     bpair = [k for k in b if "batch_encode_pair_kernel" in k]
     assert len(bpair) == 2   # with and without tile tickets
     for k in bpair:
-        assert b[k] == [0, 1, 1, 1, 0, 0], (k, b[k])
+        ticket = "batch_encode_pair_kernelILb1E" in k
+        assert b[k] == ([0] if ticket else []) + [0, 1, 1, 1, 0, 0], (k, b[k])
     fused += bpair
     # single-wave workgroups (one-wave encoders, one-wave batched encoder) need no barrier instruction
     assert not [k for k in b if "wave" in k or ("batch_encode" in k and "pair" not in k)]

@@ -76,6 +76,10 @@ def test_sizes_and_messages():
         assert lib.sleek_gpu_strerror(st)
     assert lib.sleek_gpu_set_tuning(-1, 0, 0) == _lib.EINVAL
     assert lib.sleek_gpu_set_tuning(7, 0, 0) == _lib.EINVAL
+    assert lib.sleek_gpu_set_tuning(3, 0, 0) == _lib.EINVAL   # retired variant
+    assert lib.sleek_gpu_set_help_after(-1) == _lib.EINVAL
+    assert lib.sleek_gpu_set_help_after(10_000_001) == _lib.EINVAL
+    assert lib.sleek_gpu_set_help_after(100) == 0
     assert lib.sleek_gpu_set_tuning(0, -1, 0) == _lib.EINVAL
     assert lib.sleek_gpu_set_tuning(0, 0, 0) == 0
     for g, w in ((12, 4), (4, 4), (128, 4), (64, 65), (-1, 0), (8, -1)):

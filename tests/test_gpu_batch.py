@@ -69,24 +69,58 @@ def test_batch_matches_oracle(gpu, grid):
             assert dec[i].tobytes() == a.tobytes()
 
 
-@pytest.mark.parametrize("variant", [4, 5])
+def help_events():
+    from sleek_amd import _lib
+
+    n = _lib.load().sleek_gpu_help_events()
+    assert n >= 0, n
+    return n
+
+
+@pytest.mark.parametrize("variant", [4, 5, 6])
 def test_batch_encoder_forms_match_oracle(gpu, variant):
-    """The batched encoder's one-wave form (encoder variant 4) and two-wave form
-    (variants 5 and 6, the default) write the oracle's streams, at the automatic
-    grid and at 5 workgroups."""
+    """The batched encoder's one-wave form (encoder variant 4) and two-wave forms
+    (variant 5 in launch order, 6 = the default with tile tickets) write the
+    oracle's streams, at the automatic grid and at 1, 3 and 5 workgroups.  With
+    tile tickets every tile goes to a running workgroup: no look-back help."""
     from sleek_amd import _lib
 
     lib = _lib.load()
     arrays, ebs = mixed_arrays(17, 25)
     try:
-        for grid in (0, 5):
+        for grid in (0, 1, 3, 5):
             _lib.check(lib.sleek_gpu_set_tuning(variant, 0, 0), "tuning")
             say(f"batch encoder variant={variant}, {len(arrays)} arrays, grid={grid}")
+            h0 = help_events()
             streams, dec, _, st_c, st_d = run_batch(arrays, ebs, grid)
             assert not st_c.any() and not st_d.any()
             for i, (a, eb) in enumerate(zip(arrays, ebs)):
                 assert streams[i].tobytes() == po.compress(a, eb).tobytes(), (variant, grid, i, a.size, a.dtype, eb)
+            if variant == 6:
+                assert help_events() == h0, f"tile tickets needed look-back help at grid {grid}"
     finally:
+        lib.sleek_gpu_set_tuning(0, 0, 0)
+
+
+@pytest.mark.parametrize("variant", [4, 5, 6])
+def test_batch_lookback_help_always_on(gpu, variant):
+    """The batched encoders with the help threshold at 0: look-backs size missing
+    tiles and groups from the inputs themselves, and the bytes stay the oracle's."""
+    from sleek_amd import _lib
+
+    lib = _lib.load()
+    arrays, ebs = mixed_arrays(19, 12)
+    try:
+        _lib.check(lib.sleek_gpu_set_help_after(0), "help")
+        _lib.check(lib.sleek_gpu_set_tuning(variant, 0, 0), "tuning")
+        for grid in (0, 3):
+            say(f"batch help always on, variant={variant}, grid={grid}")
+            streams, _, _, st_c, st_d = run_batch(arrays, ebs, grid)
+            assert not st_c.any() and not st_d.any()
+            for i, (a, eb) in enumerate(zip(arrays, ebs)):
+                assert streams[i].tobytes() == po.compress(a, eb).tobytes(), (variant, grid, i, a.size, a.dtype, eb)
+    finally:
+        lib.sleek_gpu_set_help_after(100)
         lib.sleek_gpu_set_tuning(0, 0, 0)
 
 

@@ -104,13 +104,22 @@ def test_blocks_invariance(gpu, dt, blocks):
             assert rc == 0 and d.tobytes() == od.tobytes()
 
 
+def help_events():
+    from sleek_amd import _lib
+
+    n = _lib.load().sleek_gpu_help_events()
+    assert n >= 0, n
+    return n
+
+
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6])
 def test_encoder_variant_invariance(gpu, dt, variant):
     """Every encoder kernel (sleek_gpu_set_tuning's encode_variant) writes the oracle's bytes,
-    at the automatic grid and at a 5-workgroup grid that makes each one walk many chunks
-    (variant 6: three of the eight chunk-ticket sets then have no workgroup of their own,
-    so their chunks are all taken by the fallback to other sets)."""
+    at the automatic grid and at grids of 1, 3, 5 and 8 workgroups that make each one walk
+    many chunks (the ticket encoder then runs fewer ticket sets than eight, one per
+    workgroup).  The ticket encoder (0 = 6) hands every chunk to a running workgroup, so
+    its look-backs never need to size a missing chunk themselves: no help event."""
     say(f"encoder variant={variant} {np.dtype(dt).name}")
     from sleek_amd import _lib
     from sleek_amd.datagen import exact_field, inject_specials
@@ -119,16 +128,47 @@ def test_encoder_variant_invariance(gpu, dt, variant):
     a = exact_field(400_003 if dt == np.float32 else 200_003, 5, dt)
     inject_specials(a, 1e-3, 11)
     try:
-        for grid in (0, 5):
+        for grid in (0, 1, 3, 5, 8):
             _lib.check(lib.sleek_gpu_set_tuning(variant, grid, 0), "tuning")
+            h0 = help_events()
             for eb in (None, 1e-2, 1e-5):
                 ref = po.compress(a, eb)
                 s, d = gpu_roundtrip(a, eb)
                 assert s.tobytes() == ref.tobytes(), (grid, eb)
                 od, rc = po.decompress(ref, dt, eb)
                 assert rc == 0 and d.tobytes() == od.tobytes()
+            if variant in (0, 6):
+                assert help_events() == h0, f"ticket encoder needed look-back help at grid {grid}"
     finally:
         lib.sleek_gpu_set_tuning(0, 0, 0)
+
+
+@pytest.mark.parametrize("variant", [0, 1, 4, 5])
+def test_lookback_help_always_on(gpu, variant):
+    """With the help threshold at 0 every look-back that does not settle at its first
+    poll sizes the missing chunks and groups from the input itself (wave_chunk_size,
+    wave_group_size) and publishes them next to their owners: the bytes stay the
+    oracle's, and the help path is counted."""
+    say(f"look-back help always on, encoder variant={variant}")
+    from sleek_amd import _lib
+    from sleek_amd.datagen import exact_field, inject_specials
+
+    lib = _lib.load()
+    h0 = help_events()
+    try:
+        _lib.check(lib.sleek_gpu_set_help_after(0), "help")
+        for dt, n in ((np.float32, 4096 * 300 + 7), (np.float64, 2048 * 150 + 3)):
+            a = exact_field(n, 17, dt)
+            inject_specials(a, 1e-3, 3)
+            for grid in (0, 3):
+                _lib.check(lib.sleek_gpu_set_tuning(variant, grid, 0), "tuning")
+                for eb in (None, 1e-3):
+                    s, _ = gpu_roundtrip(a, eb)
+                    assert s.tobytes() == po.compress(a, eb).tobytes(), (np.dtype(dt).name, grid, eb)
+    finally:
+        lib.sleek_gpu_set_help_after(100)
+        lib.sleek_gpu_set_tuning(0, 0, 0)
+    say(f"help events: {help_events() - h0}")
 
 
 @pytest.mark.parametrize("variant", [0, 1])
@@ -158,7 +198,7 @@ def test_decoder_variant_invariance(gpu, variant):
         lib.sleek_gpu_set_tuning(0, 0, 0)
 
 
-@pytest.mark.parametrize("variant", [1, 3, 4, 5, 6])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6])
 def test_lookback_geometry_invariance(gpu, variant):
     """Look-back group sizes 8..64 and initial windows 1..64 (sleek_gpu_set_lookback)
     change no byte, at the automatic grid and at a 3-workgroup grid."""

@@ -37,8 +37,9 @@ def parse():
     p.add_argument("--cpu-mb", type=int, default=64, help="CPU baseline sample size in MiB")
     p.add_argument("--seed", type=int, default=1)
     p.add_argument("--encode-variant", type=int, default=0,
-                   help="0 default, 1 pipelined 256-thread, 2 the same at the compiler's register choice, 3 one "
-                        "wave per chunk, 4 one wave per chunk pipelined over its chunks")
+                   help="0 default (6), 1 pipelined 256-thread, 2 the same at the compiler's register choice, "
+                        "4 one wave per chunk pipelined over its chunks, 5 two waves per chunk, 6 two waves per "
+                        "chunk with chunk tickets")
     p.add_argument("--encode-grid", type=int, default=0, help="encoder persistent grid (0 = automatic)")
     p.add_argument("--decode-grid", type=int, default=0, help="decoder grid (0 = one workgroup per chunk)")
     p.add_argument("--decoder", type=int, default=-1, help="decoder variant (-1 = library default)")
@@ -334,6 +335,16 @@ def measured_traffic(kernel, shape_scale):
                                                   "kernel": t["kernel"], "source": "profiles/traffic_512cubed_f32.json"}
 
 
+def help_events():
+    """Look-back help events on the device so far (missing look-back words a
+    waiting wave sized itself; the ticket encoders should need none)."""
+    from sleek_amd import _lib
+
+    n = _lib.load().sleek_gpu_help_events()
+    _lib.check(0 if n >= 0 else int(n), "help events")
+    return int(n)
+
+
 def run_modes(x, ebs, labels, steps, warmup):
     """Compress then decompress x once per error bound per step, with HIP
     events on the launch stream around every call.  Returns the per-mode
@@ -351,26 +362,29 @@ def run_modes(x, ebs, labels, steps, warmup):
     encs = [torch.zeros(1, dtype=torch.int64, device=dev) for _ in ebs]
     decs = [torch.empty_like(x) for _ in ebs]
     dsz = torch.zeros(1, dtype=torch.int64, device=dev)
-    ws = codec.Workspace(nbytes, dev)
+    # one workspace per timed call: each decode's status word (the call's own,
+    # cleared by its first kernel) is read back after the timed region
+    wss = [[codec.Workspace(nbytes, dev) for _ in ebs] for _ in range(max(steps, 1))]
 
-    def step(events=None):
+    def step(k, events=None):
         for i, eb in enumerate(ebs):
             if events is not None:
                 events[i][0].record(stream)
-            codec.compress(x, eb, out=bufs[i], encsize=encs[i], workspace=ws)
+            codec.compress(x, eb, out=bufs[i], encsize=encs[i], workspace=wss[k][i])
             if events is not None:
                 events[i][1].record(stream)
-            codec.decompress(bufs[i], x.dtype, eb, out=decs[i], decsize=dsz, workspace=ws)
+            codec.decompress(bufs[i], x.dtype, eb, out=decs[i], decsize=dsz, workspace=wss[k][i])
             if events is not None:
                 events[i][2].record(stream)
 
     for _ in range(warmup):
-        step()
-    torch.cuda.synchronize()
+        step(0)
+        torch.cuda.synchronize()
+        assert all(w.status() == 0 for w in wss[0]), "warm-up call status"
     iv = torch.int32 if x.element_size() == 4 else torch.int64
     assert torch.equal(decs[0].view(iv), x.view(iv)), "lossless round trip mismatch"
-    assert ws.status() == 0
     enc_sizes = [int(e.item()) for e in encs]
+    want = [d.clone() for d in decs]
 
     ev = [[[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in ebs] for _ in range(steps)]
     torch.cuda.synchronize()
@@ -379,11 +393,16 @@ def run_modes(x, ebs, labels, steps, warmup):
     end = torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for k in range(steps):
-        step(ev[k])
+        step(k, ev[k])
     end.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     t_total = start.elapsed_time(end) / 1e3
+    bad = [(k, i) for k in range(steps) for i in range(len(ebs)) if wss[k][i].status() != 0]
+    assert not bad, f"timed calls with a bad status: {bad[:8]}"
+    assert [int(e.item()) for e in encs] == enc_sizes, "stream sizes changed between steps"
+    for i in range(len(ebs)):
+        assert torch.equal(decs[i].view(iv), want[i].view(iv)), f"mode {labels[i]}: decode changed between steps"
 
     per = []
     for i, eb in enumerate(ebs):
@@ -397,7 +416,7 @@ def run_modes(x, ebs, labels, steps, warmup):
             "compress_alg_GBps": round(alg / tc / 1e9, 1), "decompress_alg_GBps": round(alg / td / 1e9, 1),
             "compress_hbm_frac": round(alg / tc / HBM_PEAK, 4), "decompress_hbm_frac": round(alg / td / HBM_PEAK, 4),
         })
-    del bufs, decs
+    del bufs, decs, want, wss
     return per, enc_sizes, t_total, wall
 
 
@@ -530,7 +549,9 @@ def main():
         for cfg, te, td in trace:
             print(f"autotune {json.dumps(cfg)} encode {te * 1e3:.4f} ms decode {td * 1e3:.4f} ms (all modes)",
                   file=sys.stderr, flush=True)
+    h0 = help_events()
     per, enc_sizes, t_total, wall = run_modes(x, ebs, labels, args.steps, args.warmup)
+    helps = {"plain": help_events() - h0}
     ms_step = t_total / args.steps * 1e3
     stream = torch.cuda.current_stream()
 
@@ -575,16 +596,25 @@ def main():
         "wall_s": round(wall, 3),
     }
     if not args.no_transfer:
+        h0 = help_events()
         out["transfer"] = bench_transfer(x, ebs[2])
+        helps["transfer"] = help_events() - h0
     x_host = None if args.no_cpu else x.cpu().numpy()
     del x
     torch.cuda.empty_cache()
     if not args.no_f64:
+        h0 = help_events()
         out["float64"] = bench_f64(args, dev)
+        helps["float64"] = help_events() - h0
     if not args.no_batch:
+        h0 = help_events()
         out["batched"] = bench_batch(args, dev)
+        helps["batched"] = help_events() - h0
     if not args.no_serve:
+        h0 = help_events()
         out["serve"] = bench_serve(args, dev)
+        helps["serve"] = help_events() - h0
+    out["lookback_help_events"] = helps
     if x_host is not None:
         out["cpu_baseline"] = cpu_baseline(x_host, args.cpu_mb)
     print(json.dumps(out), flush=True)

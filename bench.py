@@ -34,7 +34,6 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--scale", type=float, default=1.0, help="shrink every axis (tests)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    p.add_argument("--cpu-mb", type=int, default=64, help="CPU baseline sample size in MiB")
     p.add_argument("--seed", type=int, default=1)
     p.add_argument("--encode-variant", type=int, default=0,
                    help="0 default (6), 1 pipelined 256-thread, 2 the same at the compiler's register choice, "
@@ -52,44 +51,61 @@ def parse():
     p.add_argument("--batch-steps", type=int, default=3)
     p.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     p.add_argument("--serve-requests", type=int, default=2000)
-    p.add_argument("--serve-rate", type=float, default=2000.0, help="Poisson arrival rate, requests/s")
+    p.add_argument("--serve-rates", default="2000,10000,50000",
+                   help="Poisson arrival rates (requests/s) of the serving sweep; the first is the headline rate")
     p.add_argument("--serve-max-mb", type=float, default=256.0)
     p.add_argument("--serve-budget-gb", type=float, default=96.0)
     return p.parse_args()
 
 
-def cpu_baseline(a_np, mb):
-    """Reference CPU codec (oracle/_ref) if built, else the oracle restatement,
-    on a bounded leading sample, all host cores; lossless compress+decompress."""
+def cpu_baseline(fields):
+    """The reference CPU codec (oracle/_ref, compiled from the reference's own
+    headers, OpenMP on every host core) if built, else the oracle
+    restatement, on whole benchmark fields.  fields: (label, host array,
+    [(mode label, absolute eb or None)]).  Per mode: one warm-up call, then
+    the median of 3 timed calls per direction (steady clock around the codec
+    call only, as sleek_single_compressor.cpp:88-96).  Rates are insize / t."""
     from oracle import pyoracle as po
 
-    n = min(a_np.size, (mb << 20) // a_np.itemsize)
-    sample = np.ascontiguousarray(a_np[:n])
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     use_ref = po.ref() is not None
-    comp = (lambda x: po.ref_compress(x)) if use_ref else (lambda x: po.compress(x))
-    dec = (lambda s: po.ref_decompress(s, sample.dtype)) if use_ref else (lambda s: po.decompress(s, sample.dtype)[0])
-    s = comp(sample)
-    dec(s)  # warm-up
-    tc, td = [], []
-    for _ in range(3):
-        t0 = time.perf_counter()
-        s = comp(sample)
-        t1 = time.perf_counter()
-        dec(s)
-        t2 = time.perf_counter()
-        tc.append(t1 - t0)
-        td.append(t2 - t1)
-    tcm, tdm = float(np.median(tc)), float(np.median(td))
+    rows = []
+    for label, a, modes in fields:
+        a = np.ascontiguousarray(a)
+        for mode, eb in modes:
+            comp = (lambda x: po.ref_compress(x, eb)) if use_ref else (lambda x: po.compress(x, eb))
+            dec = (lambda s: po.ref_decompress(s, a.dtype, eb)) if use_ref else \
+                (lambda s: po.decompress(s, a.dtype, eb)[0])
+            st = comp(a)
+            dec(st)  # warm-up
+            tc, td = [], []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                st = comp(a)
+                t1 = time.perf_counter()
+                dec(st)
+                t2 = time.perf_counter()
+                tc.append(t1 - t0)
+                td.append(t2 - t1)
+            tcm, tdm = float(np.median(tc)), float(np.median(td))
+            rows.append({"field": label, "mode": mode, "bytes": a.nbytes, "ratio": round(a.nbytes / st.size, 4),
+                         "compress_GBps": round(a.nbytes / tcm / 1e9, 3),
+                         "decompress_GBps": round(a.nbytes / tdm / 1e9, 3)})
+            print(f"cpu baseline: {label} {mode}: {rows[-1]['compress_GBps']} / {rows[-1]['decompress_GBps']} GB/s",
+                  file=sys.stderr, flush=True)
+    head = rows[0]
     return {
-        "value": round(2 * sample.nbytes / (tcm + tdm) / 1e9, 3),
+        "value": round(2 / (1 / head["compress_GBps"] + 1 / head["decompress_GBps"]), 3),
         "unit": "GB/s",
         "cores": cores,
         "kind": "reference" if use_ref else "port",
-        "sample": f"first {sample.nbytes >> 20} MiB of the same field, lossless compress+decompress, "
-                  f"median of 3 after 1 warm-up, OMP threads={os.environ.get('OMP_NUM_THREADS', cores)}",
-        "compress_GBps": round(sample.nbytes / tcm / 1e9, 3),
-        "decompress_GBps": round(sample.nbytes / tdm / 1e9, 3),
+        "sample": f"whole fields: {', '.join(sorted(set(r['field'] for r in rows), key=[f[0] for f in fields].index))}; "
+                  "value = the 256^3 float32 field lossless, compress+decompress (2 x insize / total time); one "
+                  f"warm-up then the median of 3 calls per direction; OMP threads="
+                  f"{os.environ.get('OMP_NUM_THREADS', cores)}",
+        "compress_GBps": head["compress_GBps"],
+        "decompress_GBps": head["decompress_GBps"],
+        "rows": rows,
     }
 
 
@@ -104,7 +120,7 @@ def bench_batch(args, dev):
     import torch
 
     from sleek_amd import _lib, codec
-    from sleek_amd.batch import plan_compress, plan_decompress
+    from sleek_amd.batch import ITEM, fill_item, plan_compress, plan_decompress
     from sleek_amd.datagen import array_field, batch_sizes, finite_range
 
     lib = _lib.load()
@@ -157,24 +173,46 @@ def bench_batch(args, dev):
     t_bc = timed(gc.launch, args.batch_steps)
     t_bd = timed(gd.launch, args.batch_steps)
     t_bc_eager = timed(lambda: bc.launch(), args.batch_steps)
+    t_bd_eager = timed(lambda: bd.launch(), args.batch_steps)
 
-    # per-array plain launches through the C ABI
-    maxb = max(x.numel() * x.element_size() for x in xs)
-    ws = codec.Workspace(maxb, dev)
+    # per-array plain launches through the C ABI, each call with its own
+    # workspace (256-byte aligned slices of one buffer) so that every call's
+    # status is read back: from a Python ctypes loop and from a C++ loop
+    # (sleek_amd/cli/bin/libsleek_plain_each.so)
     pouts = [torch.empty_like(o) for o in bc.outs]
     psz = torch.zeros(len(xs), dtype=torch.int64, device=dev)
     pdec = [torch.empty_like(x) for x in xs]
     pdsz = torch.zeros(len(xs), dtype=torch.int64, device=dev)
+    wsb = np.array([codec.workspace_size(x.numel() * x.element_size()) for x in xs], dtype=np.int64)
+    wsoff = np.concatenate([[0], np.cumsum((wsb + 255) // 256 * 256)])
+    wsbuf = torch.empty(int(wsoff[-1]), dtype=torch.uint8, device=dev)
+    wsptr = np.array([wsbuf.data_ptr() + int(o) for o in wsoff[:-1]], dtype=np.uint64)
+    citems = np.zeros(len(xs), dtype=ITEM)
+    ditems = np.zeros(len(xs), dtype=ITEM)
     cargs, dargs = [], []
     for i, (x, eb) in enumerate(zip(xs, ebs)):
         eb_b = x.element_size()
         e, t, o = (0, 0, 0) if eb is None else codec.error_bound(eb, x.dtype)
         nb = x.numel() * eb_b
         cargs.append((eb_b, int(eb is not None), e, t, o, x.data_ptr(), nb, pouts[i].data_ptr(),
-                      psz.data_ptr() + 8 * i, ws.ptr(), ws.nbytes, 0, h))
+                      psz.data_ptr() + 8 * i, int(wsptr[i]), int(wsb[i]), 0, h))
         dargs.append((eb_b, int(eb is not None), t, o, streams[i].data_ptr(), int(sizes[i]), pdec[i].data_ptr(), nb,
-                      pdsz.data_ptr() + 8 * i, ws.ptr(), ws.nbytes, 0, h))
+                      pdsz.data_ptr() + 8 * i, int(wsptr[i]), int(wsb[i]), 0, h))
+        for rec, src, dst, sb, db, dsz_p in ((citems[i], x.data_ptr(), pouts[i].data_ptr(), nb, pouts[i].numel(),
+                                              psz.data_ptr() + 8 * i),
+                                             (ditems[i], streams[i].data_ptr(), pdec[i].data_ptr(), int(sizes[i]), nb,
+                                              pdsz.data_ptr() + 8 * i)):
+            rec["src"], rec["dst"], rec["src_bytes"], rec["dst_bytes"], rec["d_size"] = src, dst, sb, db, dsz_p
+            fill_item(rec, x.dtype, eb)
     comp, decomp = lib.sleek_gpu_compress_async, lib.sleek_gpu_decompress_async
+    each = ctypes.CDLL(os.path.join(ROOT, "sleek_amd", "cli", "bin", "libsleek_plain_each.so")).sleek_plain_each
+    each.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    wsb_c = np.ascontiguousarray(wsb)
+    status_idx = torch.from_numpy((wsoff[:-1] // 8 + 1).astype(np.int64)).to(dev)
+
+    def statuses():
+        torch.cuda.synchronize()
+        return wsbuf.view(torch.int64)[status_idx].cpu().numpy()
 
     def plain_c():
         for a in cargs:
@@ -184,15 +222,27 @@ def bench_batch(args, dev):
         for a in dargs:
             decomp(*a)
 
+    def plain_c_native():
+        _lib.check(each(0, citems.ctypes.data, len(xs), wsptr.ctypes.data, wsb_c.ctypes.data, h), "plain loop")
+
+    def plain_d_native():
+        _lib.check(each(1, ditems.ctypes.data, len(xs), wsptr.ctypes.data, wsb_c.ctypes.data, h), "plain loop")
+
     for a in cargs:
         _lib.check(comp(*a), "plain compress")
+    assert not statuses().any(), "plain compress status"
     for a in dargs:
         _lib.check(decomp(*a), "plain decompress")
-    torch.cuda.synchronize()
-    assert ws.status() == 0
+    assert not statuses().any(), "plain decompress status"
     print("batched workload: plain launches warmed up", file=sys.stderr, flush=True)
     t_pc = timed(plain_c, args.batch_steps)
+    assert not statuses().any(), "plain compress status"
+    t_pc_native = timed(plain_c_native, args.batch_steps)
+    assert not statuses().any(), "plain compress status (C++ loop)"
     t_pd = timed(plain_d, args.batch_steps)
+    assert not statuses().any(), "plain decompress status"
+    t_pd_native = timed(plain_d_native, args.batch_steps)
+    assert not statuses().any(), "plain decompress status (C++ loop)"
 
     # the two forms wrote the same bytes
     assert torch.equal(psz[: len(xs)], bc.sizes[: len(xs)]), "batched and plain stream sizes differ"
@@ -204,6 +254,7 @@ def bench_batch(args, dev):
             assert torch.equal(bd.outs[i].view(torch.uint8), xs[i].view(torch.uint8)), f"lossless array {i} differs"
     gc.close()
     gd.close()
+    del wsbuf
     enc_total = int(sizes.sum())
     out = {
         "arrays": len(xs), "input_bytes": total, "stream_bytes": enc_total,
@@ -212,14 +263,17 @@ def bench_batch(args, dev):
         "sizes": f"log-uniform 4 KiB .. {args.batch_max_mb:g} MiB",
         "tiles": bc.tiles,
         "compress": {"batched_graph_ms": round(t_bc * 1e3, 3), "batched_eager_ms": round(t_bc_eager * 1e3, 3),
-                     "plain_ms": round(t_pc * 1e3, 3),
-                     "batched_GBps": round(total / t_bc / 1e9, 1), "plain_GBps": round(total / t_pc / 1e9, 1),
-                     "speedup": round(t_pc / t_bc, 3),
+                     "plain_ms": round(t_pc_native * 1e3, 3), "plain_python_loop_ms": round(t_pc * 1e3, 3),
+                     "batched_GBps": round(total / t_bc / 1e9, 1), "plain_GBps": round(total / t_pc_native / 1e9, 1),
+                     "speedup": round(t_pc_native / t_bc, 3), "speedup_vs_python_loop": round(t_pc / t_bc, 3),
                      "batched_hbm_frac": round((total + enc_total) / t_bc / HBM_PEAK, 4)},
-        "decompress": {"batched_graph_ms": round(t_bd * 1e3, 3), "plain_ms": round(t_pd * 1e3, 3),
-                       "batched_GBps": round(total / t_bd / 1e9, 1), "plain_GBps": round(total / t_pd / 1e9, 1),
-                       "speedup": round(t_pd / t_bd, 3),
+        "decompress": {"batched_graph_ms": round(t_bd * 1e3, 3), "batched_eager_ms": round(t_bd_eager * 1e3, 3),
+                       "plain_ms": round(t_pd_native * 1e3, 3), "plain_python_loop_ms": round(t_pd * 1e3, 3),
+                       "batched_GBps": round(total / t_bd / 1e9, 1), "plain_GBps": round(total / t_pd_native / 1e9, 1),
+                       "speedup": round(t_pd_native / t_bd, 3), "speedup_vs_python_loop": round(t_pd / t_bd, 3),
                        "batched_hbm_frac": round((total + enc_total) / t_bd / HBM_PEAK, 4)},
+        "plain_loop": "one stream-ordered C ABI call per array, each with its own workspace; plain_ms from a C++ "
+                      "loop (sleek_plain_each), plain_python_loop_ms from a Python ctypes loop",
         "generate_s": round(t_gen, 2),
         "identical": True,
     }
@@ -234,8 +288,9 @@ def bench_serve(args, dev):
     lossy at {1e-2, 1e-3, 1e-4} x range, admission under a device-memory
     budget, continuous batching over 4 streams.  Inputs are resident on the
     device: compress requests read prefixes of pooled fields; decompress
-    requests read streams prepared before the run.  A sample of results is
-    checked against the same requests served alone."""
+    requests read streams prepared before the run.  The same requests are
+    served at each rate of the sweep (arrival times scaled); after each run
+    every result is compared with the same request served alone."""
     import torch
 
     from sleek_amd import codec
@@ -243,6 +298,7 @@ def bench_serve(args, dev):
     from sleek_amd.datagen import array_field, finite_range
     from sleek_amd.serve import COMPRESS, DECOMPRESS, GpuEngine, Request, Server, poisson_arrivals, summarize
 
+    rates = [float(r) for r in args.serve_rates.split(",")]
     rng = np.random.default_rng(args.seed + 91)
     hi = int(args.serve_max_mb * (1 << 20))
     pools = {}
@@ -252,7 +308,7 @@ def bench_serve(args, dev):
             f = array_field(hi, dt, seed=args.seed * 7919 + k + (100 if dt == np.float64 else 0), device=dev)
             pools[dt].append((f, finite_range(f)))
     n = args.serve_requests
-    arrivals = poisson_arrivals(n, args.serve_rate, args.seed + 92)
+    unit = poisson_arrivals(n, 1.0, args.seed + 92)   # arrivals of a rate-1 process, scaled per rate
     reqs = []
     pre_x, pre_eb, pre_idx = [], [], []
     for i in range(n):
@@ -267,9 +323,9 @@ def bench_serve(args, dev):
             rel = float(rng.choice([1e-2, 1e-3, 1e-4]))
             eb = float(np.dtype(dt).type(np.dtype(dt).type(r) * np.dtype(dt).type(rel)))
         if rng.random() < 0.5:
-            reqs.append(Request(i, COMPRESS, x, dt, eb, arrival=float(arrivals[i])))
+            reqs.append(Request(i, COMPRESS, x, dt, eb))
         else:
-            reqs.append(Request(i, DECOMPRESS, None, dt, eb, numel=m, arrival=float(arrivals[i])))
+            reqs.append(Request(i, DECOMPRESS, None, dt, eb, numel=m))
             pre_x.append(x)
             pre_eb.append(eb)
             pre_idx.append(i)
@@ -281,47 +337,53 @@ def bench_serve(args, dev):
         reqs[i].src = pb.outs[j][: int(psz[j])]
     print(f"serving workload: {n} requests prepared", file=sys.stderr, flush=True)
 
-    sample = set(range(0, n, max(1, n // 24)))
-    keep = {}
+    # each request served alone (single-array calls), the reference for every run
+    ws = codec.Workspace(hi, dev)
+    alone = {}
+    for r in reqs:
+        if r.kind == COMPRESS:
+            o, k = codec.compress(r.src, r.eb, workspace=ws)
+            alone[r.rid] = o[: int(k.item())].clone()
+        else:
+            tdt = torch.float32 if np.dtype(r.dtype) == np.float32 else torch.float64
+            alone[r.rid] = codec.decompress(r.src, tdt, r.eb, numel=r.numel, workspace=ws)[0]
+        assert ws.status() == 0
+    torch.cuda.synchronize()
 
-    def on_done(r):
-        if r.rid in sample:
-            keep[r.rid] = r.out
-        r.out = None
+    engine = GpuEngine(dev, streams=4)
 
-    def serve_once(rs, engine):
-        for r in rs:
-            r.admitted = r.done = float("nan")
-        srv = Server(engine, budget_bytes=args.serve_budget_gb * 1e9, max_inflight=4, on_done=on_done)
+    def serve_once(rs):
+        srv = Server(engine, budget_bytes=args.serve_budget_gb * 1e9, max_inflight=4)
         srv.run(rs)
         torch.cuda.synchronize()
         return srv
 
-    engine = GpuEngine(dev, streams=4)
     # warm-up pass over the first requests (allocator, pinned host buffers)
-    warm = [Request(r.rid, r.kind, r.src, r.dtype, r.eb, r.numel, r.arrival) for r in reqs[: min(64, n)]]
-    serve_once(warm, engine)
-    keep.clear()
-    srv = serve_once(reqs, engine)
-    stats = summarize(reqs, srv)
-    bad = [r.rid for r in reqs if r.status != 0]
-    assert not bad, f"requests with errors: {bad[:8]}"
-    for rid in sorted(keep):
-        r = reqs[rid]
-        if r.kind == COMPRESS:
-            alone, k = codec.compress(r.src, r.eb)
-            torch.cuda.synchronize()
-            assert torch.equal(keep[rid], alone[: int(k.item())]), f"request {rid} differs from its single call"
-        else:
-            tdt = torch.float32 if np.dtype(r.dtype) == np.float32 else torch.float64
-            alone, _ = codec.decompress(r.src, tdt, r.eb, numel=r.numel)
-            torch.cuda.synchronize()
-            assert torch.equal(keep[rid].view(torch.uint8), alone.view(torch.uint8)), f"request {rid} differs"
-    stats.update({"rate_rps": args.serve_rate, "sizes": f"log-uniform 64 KiB .. {args.serve_max_mb:g} MiB",
-                  "streams": 4, "checked_alone": len(keep), "identical": True})
-    del pools, reqs, pb, keep
+    serve_once([Request(r.rid, r.kind, r.src, r.dtype, r.eb, r.numel, float(unit[r.rid]) / rates[0])
+                for r in reqs[: min(64, n)]])
+    sweep = []
+    for rate in rates:
+        run = [Request(r.rid, r.kind, r.src, r.dtype, r.eb, r.numel, float(unit[r.rid]) / rate) for r in reqs]
+        srv = serve_once(run)
+        stats = summarize(run, srv)
+        bad = [r.rid for r in run if r.status != 0]
+        assert not bad, f"requests with errors at {rate:g}/s: {bad[:8]}"
+        for r in run:
+            ref = alone[r.rid]
+            same = torch.equal(r.out, ref) if r.kind == COMPRESS else \
+                torch.equal(r.out.view(torch.uint8), ref.view(torch.uint8))
+            assert same, f"request {r.rid} at {rate:g}/s differs from its single call"
+        stats.update({"rate_rps": rate, "checked_alone": len(run), "identical": True})
+        sweep.append(stats)
+        print(f"serving: {rate:g}/s -> {stats['GBps']} GB/s, p50 {stats['p50_ms']} ms, p99 {stats['p99_ms']} ms, "
+              f"mean batch {stats['mean_batch']}", file=sys.stderr, flush=True)
+        del run
+    out = dict(sweep[0])
+    out.update({"sizes": f"log-uniform 64 KiB .. {args.serve_max_mb:g} MiB", "streams": 4, "sweep": sweep,
+                "saturation": max(sweep, key=lambda s: s["GBps"] or 0)})
+    del pools, reqs, pb, alone
     torch.cuda.empty_cache()
-    return stats
+    return out
 
 
 def measured_traffic(kernel, shape_scale):
@@ -506,8 +568,9 @@ def bench_f64(args, dev):
     out = {"workload": "384^3 float64, 12 sinusoids + lognormal(-10, 2) noise, 0.01% specials",
            "bytes": nbytes, "per_mode": per, "memcpy_d2d": memcpy_d2d(x),
            "GBps": round(2 * nbytes * len(ebs) / (t_total / max(3, args.steps // 2)) / 1e9, 2)}
+    host = None if args.no_cpu else (x.cpu().numpy(), list(zip(labels, ebs)))
     del x
-    return out
+    return out, host
 
 
 def main():
@@ -599,13 +662,20 @@ def main():
         h0 = help_events()
         out["transfer"] = bench_transfer(x, ebs[2])
         helps["transfer"] = help_events() - h0
-    x_host = None if args.no_cpu else x.cpu().numpy()
+    cpu_fields = []
+    if not args.no_cpu:
+        f0 = config_field(0, seed=args.seed, device=dev, scale=args.scale).reshape(-1).cpu().numpy()
+        cpu_fields.append((f"{f0.size}-value float32 (256^3 field, 8 sinusoids + noise)", f0, [("lossless", None)]))
+        cpu_fields.append((f"{x.numel()}-value float32 (the 512^3 field above)", x.cpu().numpy(),
+                           list(zip(labels, ebs))))
     del x
     torch.cuda.empty_cache()
     if not args.no_f64:
         h0 = help_events()
-        out["float64"] = bench_f64(args, dev)
+        out["float64"], f64_host = bench_f64(args, dev)
         helps["float64"] = help_events() - h0
+        if f64_host is not None:
+            cpu_fields.append((f"{f64_host[0].size}-value float64 (384^3 field)", f64_host[0], f64_host[1]))
     if not args.no_batch:
         h0 = help_events()
         out["batched"] = bench_batch(args, dev)
@@ -615,8 +685,9 @@ def main():
         out["serve"] = bench_serve(args, dev)
         helps["serve"] = help_events() - h0
     out["lookback_help_events"] = helps
-    if x_host is not None:
-        out["cpu_baseline"] = cpu_baseline(x_host, args.cpu_mb)
+    if cpu_fields:
+        out["cpu_baseline"] = cpu_baseline(cpu_fields)
+        del cpu_fields
     print(json.dumps(out), flush=True)
 
 

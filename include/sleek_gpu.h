@@ -106,17 +106,22 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
  * 4-waves-per-SIMD register budget, 2 = the same at the compiler's register
  * choice, 4 = one wave per chunk pipelined over the wave's chunks, 5 = two
  * waves per chunk pipelined the same way, 6 = 5 with every chunk handed out
- * in arrival order by ticket counters (3, an unpipelined one-wave encoder,
- * is retired); encode_grid / decode_grid: grid sizes (0 = automatic).  They
+ * in arrival order by ticket counters, 7 = 4 with chunk tickets (3, an
+ * unpipelined one-wave encoder, is retired).  The batched calls use two
+ * waves per tile with tile tickets, except under variant 4 (one wave per
+ * tile, tiles in launch order).  encode_grid / decode_grid: grid sizes
+ * (0 = automatic).  They
  * change speed only, never the bytes.  Applies to calls made after it
  * returns; SLEEK_EINVAL for an unknown variant or a negative size. */
 int sleek_gpu_set_tuning(int encode_variant, int encode_grid, int decode_grid);
 /* Look-back help (no reference counterpart; the reference's look-back only
- * waits, sleek_common_gpu.h:220-253): a look-back that has waited this many
- * microseconds for a predecessor's size computes it from the input itself
- * (0 = at once, for tests; default 100; at most 10^7).  The ticket encoder
- * (variant 6) hands chunks to running workgroups only and should never
- * need it.  Speed only, never the bytes; applies to later calls. */
+ * waits, sleek_common_gpu.h:220-253): a look-back that has waited about this
+ * many microseconds for a predecessor's size (its own sleep time: time its
+ * queue spends suspended does not count) computes it from the input itself
+ * (0 = at once, for tests; default 100; at most 10^7).  The ticket encoders
+ * (variants 6 and 7, and the batched encoder except under variant 4) hand
+ * chunks to running workgroups only and should never need it.  Speed only,
+ * never the bytes; applies to later calls. */
 int sleek_gpu_set_help_after(int microseconds);
 /* Help events (missing look-back words a waiting wave sized itself) on the
  * current device since the library was loaded; synchronises the device.

@@ -3,10 +3,10 @@
 The stream format fixes the chunk (16 KiB) and everything the bytes depend
 on; the tuner searches only what changes speed:
 
-* encoder workgroup shape: two 64-thread waves per chunk (with or without
-  chunk tickets), one wave per chunk, or the pipelined 256-thread encoder,
-  and its persistent grid (a fraction of the resident grid: a larger grid
-  only adds look-back waits, and the library caps it);
+* encoder workgroup shape among the ticket encoders: two 64-thread waves per
+  chunk (128-thread workgroups) or one wave per chunk, and its persistent
+  grid (a fraction of the resident grid: a larger grid only adds look-back
+  waits, and the library caps it);
 * look-back geometry: chunks per look-back group (8..64) and the window of
   groups the first group-level poll watches (1..64);
 * decoder: offset scan + one workgroup per chunk, or the persistent decoder
@@ -27,7 +27,10 @@ import os
 DEFAULT = {"encode_variant": 0, "encode_grid_mult": 0.0, "lb_group": 32, "lb_window": 0, "decoder": 0,
            "decode_grid_mult": 0.0}   # 0: the library's own choice (lb_window: by encoder)
 
-ENCODE_SHAPES = [(0, 0.0), (5, 0.0), (4, 0.0), (6, 0.75), (1, 0.0)]
+# ticket encoders only: they hand chunks to running workgroups, so their
+# look-backs never wait on a workgroup that has not started (two waves per
+# chunk = 0/6, one wave per chunk = 7; grid multiples of the resident grid)
+ENCODE_SHAPES = [(0, 0.0), (7, 0.0), (6, 0.75), (7, 0.75)]
 MIN_GAIN = 0.01   # a candidate must beat the best so far by 1 % (timing noise is about that)
 CONFIRM = 3       # a winner is timed again this often, alternately with the default, before it is kept
 # knob groups confirmed one by one: (keys, 0 = timed by the encoder / 1 = by the decoder)

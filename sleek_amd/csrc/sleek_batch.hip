@@ -18,10 +18,13 @@ namespace sleek {
 
 // One workgroup per array: clear its workspace block (16-byte stores),
 // write its tiles into the map, and give an empty array to be compressed
-// its 8-byte zero header (it has no tile).
+// its 8-byte zero header (it has no tile).  Workgroup 0 also clears the tile
+// tickets when there are any.
 __global__ __launch_bounds__(256) void batch_prep_kernel(const BatchItem* items, uint32_t* tile_map, uint64_t* ws,
-                                                         int compress)
+                                                         int compress, uint64_t* tickets)
 {
+  if (tickets && blockIdx.x == 0)
+    for (int k = threadIdx.x; k < kTicketWords / 2; k += 256) ((uint4*)tickets)[k] = make_uint4(0u, 0u, 0u, 0u);
   const BatchItem* it = items + blockIdx.x;
   const long long w0 = it->ws_word, half = it->ws_words / 2, t0 = it->tile0, nt = it->tiles;
   uint4* const z = (uint4*)(ws + w0);
@@ -50,7 +53,7 @@ struct EncTile {
 struct LookCfg {
   int lb_shift, lb_window;
   unsigned long long* helps;
-  unsigned int help_ticks;
+  unsigned int help_naps;
 };
 
 __device__ __forceinline__ EncTile enc_tile(const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map,
@@ -70,7 +73,7 @@ __device__ __forceinline__ EncTile enc_tile(const BatchItem* __restrict__ items,
   e.a.lb_window = lc.lb_window;
   e.a.tickets = nullptr;
   e.a.helps = lc.helps;
-  e.a.help_ticks = lc.help_ticks;
+  e.a.help_naps = lc.help_naps;
   e.nchunks = (e.a.insize + kChunk - 1) / kChunk;
   e.c = t - it->tile0;
   e.kind = item_kind(it);
@@ -191,15 +194,15 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
 // tile, as in the single-array two-wave encoder, and the next tile loads and
 // is coded while the current one is packed, looked back and stored.  The
 // last tile is finished after the loop, so the loop body has no branch on
-// whether a next tile exists.  TICKET: a workgroup takes every tile, its
-// first included, from the tile tickets, as the single-array ticket encoder
-// takes chunks, so an array's tiles start in arrival order and no tile waits
-// on a workgroup that has not started.
-template <bool TICKET>
+// whether a next tile exists.  A workgroup takes every tile, its first
+// included, from the tile tickets, as the single-array ticket encoder takes
+// chunks, so an array's tiles start in arrival order and no tile waits on a
+// workgroup that has not started.
 __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) void batch_encode_pair_kernel(
     const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map, long long tiles, uint64_t* ws,
     LookCfg lc, uint32_t* tickets)
 {
+  constexpr bool TICKET = true;
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   __shared__ uint32_t xl[2][8];
   __shared__ unsigned long long xexcl;
@@ -207,9 +210,10 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const int h = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   long long t = blockIdx.x;
+  const Tickets tk = ticket_setup(tickets, tiles);
   if (TICKET) {
     if (h == 0) {
-      const long long m = ticket_chunk(tickets, ticket_request(tickets), tiles);
+      const long long m = ticket_chunk(tk, ticket_request(tk), tiles);
       if (lane == 0) xfirst = m;
     }
     __syncthreads();
@@ -218,7 +222,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   }
   if (t >= tiles) return;
   uint32_t tq = 0;
-  if (TICKET && h == 0) tq = ticket_request(tickets);
+  if (TICKET && h == 0) tq = ticket_request(tk);
   EncTile ec = enc_tile(items, tile_map, t, ws, lc);
   uint2 cur[kHalfSub];
   uint32_t lh[4];
@@ -229,7 +233,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     for (int i = 0; i < 4; i++) xl[0][4 * h + i] = lh[i];
   }
   if (TICKET && h == 0) {
-    const long long m = ticket_chunk(tickets, tq, tiles);
+    const long long m = ticket_chunk(tk, tq, tiles);
     if (lane == 0) xnext = m;
   }
   __syncthreads();
@@ -238,7 +242,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   int par = 1;
   long long n = TICKET ? uni(xnext) : t + gridDim.x;
   while (TICKET ? n >= 0 : n < tiles) {
-    if (TICKET && h == 0) tq = ticket_request(tickets);
+    if (TICKET && h == 0) tq = ticket_request(tk);
     const EncTile en = enc_tile(items, tile_map, n, ws, lc);
     uint2 nxt[kHalfSub];
     load_half<true>(en.a, en.c, h, nxt);
@@ -257,7 +261,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       __builtin_amdgcn_s_setprio(0);
       enc_settle(ec, sc.csize, &xexcl);
       if (TICKET) {
-        const long long m = ticket_chunk(tickets, tq, tiles);
+        const long long m = ticket_chunk(tk, tq, tiles);
         if (lane == 0) xnext = m;
       }
     }
@@ -354,17 +358,17 @@ int batch_resident_grid(bool decompress, bool pair)
   if (hipGetDevice(&dev) != hipSuccess) return 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
   const void* fn = decompress ? (const void*)batch_decode_kernel
-                              : (pair ? (const void*)batch_encode_pair_kernel<true> : (const void*)batch_encode_kernel);
+                              : (pair ? (const void*)batch_encode_pair_kernel : (const void*)batch_encode_kernel);
   const int threads = decompress ? kThreads : (pair ? 128 : 64);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);
 }
 
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, unsigned int help_ticks,
+                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, unsigned int help_naps,
                         hipStream_t stream)
 {
-  const LookCfg lc{lb_shift, lb_window, help_counter(), help_ticks};
+  const LookCfg lc{lb_shift, lb_window, help_counter(), help_naps};
   if (n <= 0) return hipSuccess;
   uint32_t* const map = (uint32_t*)(ws + batch_map_word(n));
   const long long res = batch_resident_grid(decompress, pair);
@@ -375,19 +379,17 @@ hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long lon
   if (!decompress && res > 0 && g > res) g = res;   // the encoder's look-back: no more than the resident set
   if (g <= 0) g = 1024;
   if (g > tiles) g = tiles > 0 ? tiles : 1;
-  hipLaunchKernelGGL(batch_prep_kernel, dim3(n), dim3(256), 0, stream, items, map, ws, decompress ? 0 : 1);
+  const bool tk = !decompress && pair && tiles > 0;
+  if (tk && !tickets) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(batch_prep_kernel, dim3(n), dim3(256), 0, stream, items, map, ws, decompress ? 0 : 1,
+                     tk ? (uint64_t*)tickets : (uint64_t*)nullptr);
   if (decompress) {
     hipLaunchKernelGGL(batch_offsets_kernel, dim3(n), dim3(kScanThreads), 0, stream, items, ws);
     if (tiles > 0)
       hipLaunchKernelGGL(batch_decode_kernel, dim3((unsigned)g), dim3(kThreads), 0, stream, items, map, tiles, ws);
-  } else if (tiles > 0 && pair && tickets) {
-    const hipError_t e = zero_launch((uint64_t*)tickets, kTicketWords, stream);   // a kernel: graph-capturable
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((batch_encode_pair_kernel<true>), dim3((unsigned)g), dim3(128), 0, stream, items, map, tiles,
-                       ws, lc, tickets);
-  } else if (tiles > 0 && pair) {
-    hipLaunchKernelGGL((batch_encode_pair_kernel<false>), dim3((unsigned)g), dim3(128), 0, stream, items, map, tiles,
-                       ws, lc, (uint32_t*)nullptr);
+  } else if (tk) {
+    hipLaunchKernelGGL(batch_encode_pair_kernel, dim3((unsigned)g), dim3(128), 0, stream, items, map, tiles, ws, lc,
+                       tickets);
   } else if (tiles > 0) {
     hipLaunchKernelGGL(batch_encode_kernel, dim3((unsigned)g), dim3(64), 0, stream, items, map, tiles, ws, lc);
   }

@@ -51,7 +51,7 @@ std::atomic<int> g_decode_grid{0};
 std::atomic<int> g_decode_variant{sleek::kDecodeScan};
 std::atomic<int> g_lb_shift{5};    // look-back groups of 32 chunks
 std::atomic<int> g_lb_window{0};   // initial look-back window, in groups (0: the encoder's default)
-std::atomic<unsigned int> g_help_ticks{sleek::kHelpTicksDefault};   // look-back help threshold (100 MHz ticks)
+std::atomic<unsigned int> g_help_naps{sleek::kHelpNapsDefault};   // look-back help threshold (naps)
 
 // Per-device scratch of the synchronous entry points (grow-only).
 struct DeviceScratch {
@@ -109,6 +109,7 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
                   int blocks, hipStream_t stream)
 {
   if ((elem_bytes != 4 && elem_bytes != 8) || insize < 0 || insize % elem_bytes != 0) return SLEEK_EINVAL;
+  if (chunks_of(insize) > 0xffffffffll) return SLEEK_EINVAL;   // chunk indices are 32-bit in the kernels' tickets
   if (!d_encoded || !d_encsize || !ws || (insize > 0 && !d_input)) return SLEEK_EINVAL;
   if (!aligned(d_input, 16) || !aligned(d_encoded, 8) || !aligned(d_encsize, 8) || !aligned(ws, 8)) return SLEEK_EALIGN;
   const long long need = ws_bytes_for(insize);
@@ -129,7 +130,7 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   a.lb_window = g_lb_window > 0 ? (int)g_lb_window : sleek::default_window(sleek::resolve_variant(g_encode_variant, elem_bytes));
   a.tickets = (unsigned int*)((uint64_t*)ws + ws_words_for(insize));
   a.helps = sleek::help_counter();
-  a.help_ticks = g_help_ticks;
+  a.help_naps = g_help_naps;
   const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
   e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_variant, stream);
   if (e != hipSuccess) return hip_fail(e, "encode launch");
@@ -373,7 +374,7 @@ int sleek_gpu_set_lookback(int group_chunks, int window_groups)
 int sleek_gpu_set_help_after(int microseconds)
 {
   if (microseconds < 0 || microseconds > 10000000) return SLEEK_EINVAL;
-  g_help_ticks = (unsigned int)microseconds * 100u;   // s_memrealtime counts at 100 MHz
+  g_help_naps = (unsigned int)microseconds * sleek::kHelpNapsPerUs;
   return SLEEK_OK;
 }
 
@@ -472,17 +473,20 @@ static int batch_async(bool decompress, const sleek_gpu_batch_item* d_items, int
   if (n == 0) return SLEEK_OK;
   if (!aligned(ws, 16) || !aligned(d_items, 8)) return SLEEK_EALIGN;
   if (ws_bytes < (sleek::batch_map_word(n) + (tiles + 1) / 2) * 8) return SLEEK_EWORKSPACE;
-  // the two-wave single-array encoders select the two-wave batched encoder,
-  // the ticket one with tile tickets (counters in the workspace's last words)
+  // the batched encoder: two waves per tile with tile tickets (counters in
+  // the workspace's last words), whatever the single-array variant, except
+  // that encode variant 4 selects the one-wave encoder with tiles in launch
+  // order (a test form that relies on the look-back's help under co-running
+  // kernels)
   const int rv = sleek::resolve_variant(g_encode_variant, 4);
-  const bool pair = rv == sleek::kEncodePair || rv == sleek::kEncodePairTicket;
-  if (rv == sleek::kEncodePairTicket && ws_bytes / 8 < sleek::batch_map_word(n) + (tiles + 1) / 2 + sleek::kTicketWords)
+  const bool pair = rv != sleek::kEncodeWavePipe;
+  const long long even_end = ws_bytes / 16 * 2;   // the counters end at a 16-byte boundary
+  if (!decompress && pair && even_end < sleek::batch_map_word(n) + (tiles + 1) / 2 + sleek::kTicketWords)
     return SLEEK_EWORKSPACE;
-  uint32_t* const tickets =
-      rv == sleek::kEncodePairTicket ? (uint32_t*)((uint64_t*)ws + ws_bytes / 8 - sleek::kTicketWords) : nullptr;
+  uint32_t* const tickets = (uint32_t*)((uint64_t*)ws + even_end - sleek::kTicketWords);
+  const int window = g_lb_window > 0 ? (int)g_lb_window : sleek::default_window(pair ? sleek::kEncodePairTicket : rv);
   const hipError_t e = sleek::batch_launch(decompress, (const sleek::BatchItem*)d_items, n, tiles, (uint64_t*)ws, grid,
-                                           g_lb_shift, g_lb_window > 0 ? (int)g_lb_window : sleek::default_window(rv), pair,
-                                           tickets, g_help_ticks, (hipStream_t)stream);
+                                           g_lb_shift, window, pair, tickets, g_help_naps, (hipStream_t)stream);
   return e == hipSuccess ? SLEEK_OK : hip_fail(e, "batch launch");
 }
 

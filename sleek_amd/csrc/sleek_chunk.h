@@ -138,14 +138,15 @@ __device__ __forceinline__ void count_help(unsigned long long* helps)
 // Polls back off exponentially: the words live at device scope, and many
 // waiting waves polling at full rate would compete with the data traffic.
 // Workgroups take chunks in launch or ticket order, so the words waited on
-// are normally published within a few round trips.  After a.help_ticks
-// without progress (100 us by default) the waiter computes missing
-// words from the input and publishes them, one after another without further
-// waits, until others' words arrive again; so the look-back finishes even
-// when a predecessor's workgroup is not running (an encoder with chunks in a
-// static order whose grid exceeds what is resident).  Each word sized so is
-// counted in *a.helps: the ticket encoders hand chunks only to running
-// workgroups and should never need it.
+// are normally published within a few round trips.  After a.help_naps naps
+// without progress (about 100 us of the wave's own sleep by default; time
+// the wave spends suspended with its queue does not count) the waiter
+// computes missing words from the input and publishes them, one after
+// another without further waits, until others' words arrive again; so the
+// look-back finishes even when a predecessor's workgroup is not running (an
+// encoder with chunks in a static order whose grid exceeds what is
+// resident).  Each word sized so is counted in *a.helps: the ticket encoders
+// hand chunks only to running workgroups and should never need it.
 template <typename W, bool LOSSY>
 __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long c, long long nchunks)
 {
@@ -155,7 +156,7 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
   int gwin = a.lb_window;
   bool have_partial = (q == 0);
   uint64_t partial = 0, gacc = 0;
-  uint64_t since = __builtin_amdgcn_s_memrealtime();
+  uint32_t slept = 0;   // naps since the last progress
   int naps = 1;
   bool helping = false;   // the wait timed out: size missing words from the input, one after another
   while (true) {
@@ -173,7 +174,7 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         have_partial = true;
         progressed = true;
       }
-      if (!progressed && (helping || __builtin_amdgcn_s_memrealtime() - since > a.help_ticks)) {
+      if (!progressed && (helping || slept >= a.help_naps)) {
         const uint64_t miss = __ballot(lane < q && (f >> 62) == 0);
         if (miss) {
           const long long jc = c - __ffsll((unsigned long long)miss);
@@ -212,7 +213,7 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         gwin = min(2 * gwin, 64);
         progressed = true;
       }
-      if (!progressed && (helping || __builtin_amdgcn_s_memrealtime() - since > a.help_ticks)) {
+      if (!progressed && (helping || slept >= a.help_naps)) {
         const uint64_t miss = __ballot((fg >> 62) == 0);
         if (miss) {
           const long long gm = gend - __ffsll((unsigned long long)miss);
@@ -226,13 +227,14 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
       }
     }
     if (progressed) {   // others' words arrived: wait for them again rather than help
-      since = __builtin_amdgcn_s_memrealtime();
+      slept = 0;
       naps = 1;
       helping = false;
       continue;
     }
     if (helped) continue;
-    for (int i = 0; i < naps; i++) __builtin_amdgcn_s_sleep(2);
+    for (int i = 0; i < naps; i++) __builtin_amdgcn_s_sleep(2);   // 128 clocks each
+    slept += naps;
     if (naps < 32) naps *= 2;
   }
 }
@@ -430,18 +432,35 @@ __device__ __forceinline__ void finish_chunk(const EncodeArgs& a, const ScanStat
 constexpr int kHalfSub = kSub / 2;
 
 // ---- chunk tickets (encode_pair_kernel<.., TICKET>) ----
-// S = min(kTicketSets, grid) counters, so that every set has a workgroup of
-// its own (its home set, blockIdx.x mod S); ticket t of set x is chunk
-// S * t + x.  A workgroup takes every chunk, its first included, from the
-// counters, so a chunk is only ever held by a workgroup that runs.
-__device__ __forceinline__ int ticket_sets() { return gridDim.x < (unsigned)kTicketSets ? (int)gridDim.x : kTicketSets; }
-__device__ __forceinline__ int ticket_home() { return (int)(blockIdx.x % (unsigned)ticket_sets()); }
+// S = 2^shift counters, the largest power of two <= min(kTicketSets, grid),
+// so that every set has a workgroup of its own (its home set, blockIdx.x mod
+// S); ticket t of set x is chunk S * t + x.  A workgroup takes every chunk,
+// its first included, from the counters, so a chunk is only ever held by a
+// workgroup that runs.  The set geometry and the home set's ticket count are
+// computed once per kernel (shifts and masks only: no division).
+struct Tickets {
+  uint32_t* tk;     // the counters, kTicketStride dwords apart
+  int shift;        // S = 1 << shift sets
+  int home;         // blockIdx.x mod S
+  uint32_t limit;   // tickets of the home set
+};
 
-// Tickets of set x: chunks S * t + x below nchunks.
-__device__ __forceinline__ uint32_t ticket_limit(long long nchunks, int x, int S)
+// Tickets of set x: chunks S * t + x below nchunks (nchunks < 2^32).
+__device__ __forceinline__ uint32_t ticket_limit(uint32_t nchunks, int x, int shift)
 {
-  const long long r = nchunks - x;
-  return r > 0 ? (uint32_t)((r + S - 1) / S) : 0u;
+  return nchunks > (uint32_t)x ? ((nchunks - (uint32_t)x - 1u) >> shift) + 1u : 0u;
+}
+
+__device__ __forceinline__ Tickets ticket_setup(uint32_t* tk, long long nchunks)
+{
+  Tickets k;
+  const unsigned g = gridDim.x;
+  k.tk = tk;
+  k.shift = g >= 8 ? 3 : (g >= 4 ? 2 : (g >= 2 ? 1 : 0));
+  static_assert(kTicketSets == 8, "ticket sets: 8");
+  k.home = (int)(blockIdx.x & ((1u << k.shift) - 1u));
+  k.limit = ticket_limit((uint32_t)nchunks, k.home, k.shift);
+  return k;
 }
 
 // A ticket of the workgroup's home set, in lane 0 (other lanes: 0).  The
@@ -450,11 +469,11 @@ __device__ __forceinline__ uint32_t ticket_limit(long long nchunks, int x, int S
 // wave-wide one, whose result it would wait for at once.  Callers issue it
 // before loads they wait for early (memory operations complete in issue
 // order as far as those waits are concerned).
-__device__ __forceinline__ uint32_t ticket_request(uint32_t* tk)
+__device__ __forceinline__ uint32_t ticket_request(const Tickets& k)
 {
   uint32_t t = 0;
   if ((threadIdx.x & 63) == 0) {
-    uintptr_t p = (uintptr_t)(tk + ticket_home() * kTicketStride);
+    uintptr_t p = (uintptr_t)(k.tk + k.home * kTicketStride);
     asm volatile("" : "+v"(p));
     t = __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t*)p, 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -468,39 +487,37 @@ __device__ __forceinline__ uint32_t ticket_request(uint32_t* tk)
 // spent and each ticket below a limit goes to exactly one workgroup: every
 // chunk is taken.  A set whose ticket came back spent is not tried again, so
 // the loop runs at most S - 1 times whatever the reads return.
-__device__ __noinline__ long long ticket_fallback(uint32_t* tk, long long nchunks)
+__device__ __noinline__ long long ticket_fallback(uint32_t* tk, int shift, int home, long long nchunks)
 {
   typedef __attribute__((address_space(1))) uint32_t gu32;
   gu32* const g = (gu32*)tk;
   const int lane = threadIdx.x & 63;
-  const int S = ticket_sets();
-  const int x = ticket_home();
-  uint32_t spent = 1u << x;
+  const int S = 1 << shift;
+  uint32_t spent = 1u << home;
   while (true) {
-    const int y = (x + 1 + lane) % S;
+    const int y = (home + 1 + lane) & (S - 1);
     bool open = false;
     if (lane < S - 1 && !((spent >> y) & 1u))
-      open = __hip_atomic_load(g + y * kTicketStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ticket_limit(nchunks, y, S);
+      open = __hip_atomic_load(g + y * kTicketStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
+             ticket_limit((uint32_t)nchunks, y, shift);
     const uint64_t m = __ballot(open);
     if (m == 0) return -1;
-    const int z = (x + 1 + __builtin_ctzll(m)) % S;
+    const int z = (home + 1 + __builtin_ctzll(m)) & (S - 1);
     uint32_t u = 0;
     if (lane == 0) u = __hip_atomic_fetch_add(g + z * kTicketStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     u = uni(u);
-    if (u < ticket_limit(nchunks, z, S)) return (long long)S * u + z;
+    if (u < ticket_limit((uint32_t)nchunks, z, shift)) return ((long long)u << shift) + z;
     spent |= 1u << z;
   }
 }
 
 // The chunk of a ticket from ticket_request (uniform), or -1 once every set
 // is spent.
-__device__ __forceinline__ long long ticket_chunk(uint32_t* tk, uint32_t t, long long nchunks)
+__device__ __forceinline__ long long ticket_chunk(const Tickets& k, uint32_t t, long long nchunks)
 {
-  const int S = ticket_sets();
-  const int x = ticket_home();
   t = uni(t);
-  if (t < ticket_limit(nchunks, x, S)) return (long long)S * t + x;
-  return ticket_fallback(tk, nchunks);
+  if (t < k.limit) return ((long long)t << k.shift) + k.home;
+  return ticket_fallback(k.tk, k.shift, k.home, nchunks);
 }
 
 // GLOBAL: the array pointer was read from memory (a batch item): load

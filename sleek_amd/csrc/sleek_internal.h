@@ -22,12 +22,14 @@ struct EncodeArgs {
   int lb_window;          // initial look-back window in groups (1..64)
   unsigned int* tickets;  // zeroed chunk counters of kEncodePairTicket (kTicketSets, kTicketStride apart)
   unsigned long long* helps;   // device counter of look-back help events (help_counter())
-  unsigned int help_ticks;     // a look-back waiting this long (100 MHz ticks) sizes the missing words itself
+  unsigned int help_naps;      // a look-back that has slept this many naps (128 clocks each) without
+                               // progress sizes the missing words itself
 };
 
 // Chunk tickets (kEncodePairTicket): every chunk, a workgroup's first
-// included, comes from one of S = min(kTicketSets, grid) counters (home set
-// x = blockIdx.x mod S): ticket t of set x is chunk S * t + x.  A chunk is
+// included, comes from one of S counters, S the largest power of two up to
+// min(kTicketSets, grid) (home set x = blockIdx.x mod S): ticket t of set x
+// is chunk S * t + x.  A chunk is
 // therefore only ever held by a running workgroup, and every set has one, so
 // the look-back never waits on a workgroup that has not started.  The
 // counters sit kTicketStride dwords (512 bytes) apart, after the look-back
@@ -54,11 +56,15 @@ struct DecodeArgs {
 // rejected; kEncodeWavePipe: one wave per chunk, pipelined over the wave's
 // chunks; kEncodePair: two waves per chunk (128 threads), pipelined the same
 // way; kEncodePairTicket: kEncodePair with chunks handed out in arrival
-// order (chunk tickets).  kEncodeDefault resolves to kEncodePairTicket for
-// both element types: with its default 64-group first look-back window it is
-// about 5 % faster than kEncodePair with that one's default 8; both two-wave
-// forms run 4 waves per SIMD and beat kEncodeWavePipe by 4-8 %.  Output bytes
-// never depend on the variant or on the grid size.
+// order (chunk tickets); kEncodeWaveTicket: kEncodeWavePipe with chunk
+// tickets.  kEncodeDefault resolves to kEncodePairTicket for both element
+// types: with its default 64-group first look-back window it is about 5 %
+// faster than kEncodePair with that one's default 8; both two-wave forms run
+// 4 waves per SIMD and beat kEncodeWavePipe by 4-8 %.  The ticket encoders
+// hand chunks to running workgroups only; the others take chunks in launch
+// order and rely on the look-back's help when a workgroup holding a chunk
+// has not started.  Output bytes never depend on the variant or on the grid
+// size.
 constexpr int kEncodeDefault = 0;
 constexpr int kEncodePipelined = 1;
 constexpr int kEncodeFree = 2;
@@ -66,7 +72,9 @@ constexpr int kEncodeRetired = 3;
 constexpr int kEncodeWavePipe = 4;
 constexpr int kEncodePair = 5;
 constexpr int kEncodePairTicket = 6;
-constexpr int kEncodeVariants = 7;
+constexpr int kEncodeWaveTicket = 7;
+constexpr int kEncodeVariants = 8;
+inline bool ticket_variant(int resolved) { return resolved == kEncodePairTicket || resolved == kEncodeWaveTicket; }
 inline int resolve_variant(int v, int elem_bytes)
 {
   (void)elem_bytes;   // the same choice for both element types
@@ -78,10 +86,13 @@ inline int resolve_variant(int v, int elem_bytes)
 // where the nearest inclusive group word is about one iteration (64 groups)
 // back, so that one poll usually settles it; 8 for the others (each measured
 // best for its encoder).
-inline int default_window(int resolved_variant) { return resolved_variant == kEncodePairTicket ? 64 : 8; }
+inline int default_window(int resolved_variant) { return ticket_variant(resolved_variant) ? 64 : 8; }
 
-// Default wait before a look-back sizes a missing predecessor itself: 100 us.
-constexpr unsigned int kHelpTicksDefault = 10000;
+// A look-back's naps (s_sleep 2: 128 clocks, about 53 ns at 2.4 GHz) per
+// microsecond of its own waiting, and the default wait before it sizes a
+// missing predecessor itself: about 100 us.
+constexpr unsigned int kHelpNapsPerUs = 19;
+constexpr unsigned int kHelpNapsDefault = 100 * kHelpNapsPerUs;
 // The device address of the calling device's help-event counter (null on a
 // HIP error); kernels add one per missing word they size themselves.
 unsigned long long* help_counter();
@@ -125,11 +136,11 @@ constexpr long long kBatchHead = 16;
 inline long long even_words(long long w) { return (w + 1) & ~1ll; }
 inline long long batch_map_word(int n) { return kBatchHead + even_words(n); }
 
-// pair: the two-wave batched encoder (else one wave per tile); tickets: the
-// tile-ticket counters (kTicketWords words) for the two-wave encoder, or null
-// for tiles in a static order
+// pair: the two-wave batched encoder with tile tickets (counters: tickets,
+// kTicketWords words); else the one-wave batched encoder, tiles in launch
+// order
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, unsigned int help_ticks,
+                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, unsigned int help_naps,
                         hipStream_t stream);
 hipError_t batch_status_launch(const BatchItem* items, int n, uint64_t* ws, hipStream_t stream);
 int batch_resident_grid(bool decompress, bool pair);

@@ -230,7 +230,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 // coding and publishing of n run at a raised wave priority, since other
 // waves' look-backs wait on it (2-5 % faster).  Two chunks' raw slices are
 // live at once (128 VGPRs), two waves per SIMD.
-template <typename W, bool LOSSY>
+//
+// TICKET: the wave takes every chunk from the chunk tickets, as the two-wave
+// ticket encoder does: the first before it starts, then the chunk after next,
+// requested at the start of an iteration and read after the look-back.
+template <typename W, bool LOSSY, bool TICKET>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void encode_wave2_kernel(EncodeArgs a)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
@@ -243,15 +247,20 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     return;
   }
   const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
+  const Tickets tk = ticket_setup(a.tickets, nchunks);
   long long c = blockIdx.x;
-  if (c >= nchunks) return;
+  if (TICKET) c = ticket_chunk(tk, ticket_request(tk), nchunks);   // uniform; -1: every chunk is taken
+  if (c < 0 || c >= nchunks) return;
+  uint32_t tq = 0;
+  if (TICKET) tq = ticket_request(tk);
   uint2 cur[kSub];
   load_raw(a, c, cur);
   ChunkShape sc = shape_codes<W, LOSSY>(a, c, cur);
   publish_chunk(a, st, c, nchunks, sc.csize);
+  long long n = TICKET ? ticket_chunk(tk, tq, nchunks) : c + gridDim.x;
   while (true) {
-    const long long n = c + gridDim.x;
-    const bool more = n < nchunks;
+    const bool more = TICKET ? n >= 0 : n < nchunks;
+    if (TICKET && more) tq = ticket_request(tk);
     uint2 nxt[kSub];
     if (more) load_raw(a, n, nxt);
     pack_codes<W, LOSSY>(a, c, cur, sc, stage);
@@ -266,7 +275,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     }
     finish_chunk(a, st, c, nchunks, sc.csize, stage, (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks));
     if (!more) break;
+    const long long nn = TICKET ? ticket_chunk(tk, tq, nchunks) : n + gridDim.x;
     c = n;
+    n = nn;
     sc = sn;
 #pragma unroll
     for (int s = 0; s < kSub; s++) cur[s] = nxt[s];
@@ -308,9 +319,10 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
   const long long data_base = 8 + 2 * nchunks;
   long long c = blockIdx.x;
+  const Tickets tk = ticket_setup(a.tickets, nchunks);
   if (TICKET) {
     if (h == 0) {
-      const long long m = ticket_chunk(a.tickets, ticket_request(a.tickets), nchunks);
+      const long long m = ticket_chunk(tk, ticket_request(tk), nchunks);
       if (lane == 0) xfirst = m;
     }
     __syncthreads();
@@ -321,7 +333,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   uint2 cur[kHalfSub];
   uint32_t lh[4];
   uint32_t tq = 0;
-  if (TICKET && h == 0) tq = ticket_request(a.tickets);
+  if (TICKET && h == 0) tq = ticket_request(tk);
   load_half(a, c, h, cur);
   shape_half<W, LOSSY>(a, c, h, cur, lh);
   if (lane == 0) {
@@ -329,7 +341,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     for (int i = 0; i < 4; i++) xl[0][4 * h + i] = lh[i];
   }
   if (TICKET && h == 0) {
-    const long long m = ticket_chunk(a.tickets, tq, nchunks);
+    const long long m = ticket_chunk(tk, tq, nchunks);
     if (lane == 0) xnext = m;
   }
   __syncthreads();
@@ -340,7 +352,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   while (true) {
     const bool more = TICKET ? n >= 0 : n < nchunks;
     uint2 nxt[kHalfSub];
-    if (TICKET && h == 0 && more) tq = ticket_request(a.tickets);
+    if (TICKET && h == 0 && more) tq = ticket_request(tk);
     if (more) load_half(a, n, h, nxt);
     pack_half<W, LOSSY>(a, c, h, cur, sc, stage);
     __builtin_amdgcn_s_setprio(2);   // n's size is waited on by others: as in encode_wave2_kernel
@@ -360,7 +372,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       __builtin_amdgcn_s_setprio(0);
       const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
       long long m = -1;
-      if (TICKET && more) m = ticket_chunk(a.tickets, tq, nchunks);
+      if (TICKET && more) m = ticket_chunk(tk, tq, nchunks);
       if (lane == 0) {
         if (c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)sc.csize);
         xexcl = excl;
@@ -448,7 +460,8 @@ template <typename W, bool LOSSY>
 static hipError_t launch_encode(const EncodeArgs& a, int grid, int variant, hipStream_t stream)
 {
   variant = resolve_variant(variant, (int)sizeof(W));
-  if (variant == kEncodeWavePipe) hipLaunchKernelGGL((encode_wave2_kernel<W, LOSSY>), dim3(grid), dim3(64), 0, stream, a);
+  if (variant == kEncodeWavePipe) hipLaunchKernelGGL((encode_wave2_kernel<W, LOSSY, false>), dim3(grid), dim3(64), 0, stream, a);
+  else if (variant == kEncodeWaveTicket) hipLaunchKernelGGL((encode_wave2_kernel<W, LOSSY, true>), dim3(grid), dim3(64), 0, stream, a);
   else if (variant == kEncodePair) hipLaunchKernelGGL((encode_pair_kernel<W, LOSSY, false>), dim3(grid), dim3(128), 0, stream, a);
   else if (variant == kEncodePairTicket) hipLaunchKernelGGL((encode_pair_kernel<W, LOSSY, true>), dim3(grid), dim3(128), 0, stream, a);
   else if (variant == kEncodeFree) hipLaunchKernelGGL((encode_kernel<W, LOSSY, 1>), dim3(grid), dim3(kThreads), 0, stream, a);
@@ -486,7 +499,8 @@ template <typename W, bool LOSSY>
 static const void* encode_fn(int variant)
 {
   variant = resolve_variant(variant, (int)sizeof(W));
-  if (variant == kEncodeWavePipe) return (const void*)encode_wave2_kernel<W, LOSSY>;
+  if (variant == kEncodeWavePipe) return (const void*)encode_wave2_kernel<W, LOSSY, false>;
+  if (variant == kEncodeWaveTicket) return (const void*)encode_wave2_kernel<W, LOSSY, true>;
   if (variant == kEncodePair) return (const void*)encode_pair_kernel<W, LOSSY, false>;
   if (variant == kEncodePairTicket) return (const void*)encode_pair_kernel<W, LOSSY, true>;
   return variant == kEncodeFree ? (const void*)encode_kernel<W, LOSSY, 1> : (const void*)encode_kernel<W, LOSSY, 4>;
@@ -507,7 +521,7 @@ int resident_grid(int elem_bytes, bool lossy, bool decode, int variant)
                          : (lossy ? encode_fn<uint64_t, true>(variant) : encode_fn<uint64_t, false>(variant));
   }
   const int rv = resolve_variant(variant, elem_bytes);
-  const int threads = decode ? kThreads : (rv == kEncodeWavePipe ? 64 : (rv == kEncodePair || rv == kEncodePairTicket ? 128 : kThreads));
+  const int threads = decode ? kThreads : ((rv == kEncodeWavePipe || rv == kEncodeWaveTicket) ? 64 : (rv == kEncodePair || rv == kEncodePairTicket ? 128 : kThreads));
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, threads, 0) != hipSuccess) return 0;
   return cus * (per > 0 ? per : 1);
 }

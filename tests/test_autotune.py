@@ -9,7 +9,7 @@ def fake_measure(costs, digests=None):
 
     def measure(cfg):
         calls.append(dict(cfg))
-        e = 1.0 - 0.1 * (cfg["encode_variant"] == 4) - 0.05 * (cfg["lb_window"] == 8)
+        e = 1.0 - 0.1 * (cfg["encode_variant"] == 7) - 0.05 * (cfg["lb_window"] == 8)
         d = 1.0 - 0.2 * (cfg["decode_grid_mult"] == 2.0 and cfg["decoder"] == 0)
         dig = "same" if digests is None else digests(cfg)
         return e * costs, d * costs, dig
@@ -19,7 +19,7 @@ def fake_measure(costs, digests=None):
 def test_search_picks_fastest_per_coordinate():
     measure, calls = fake_measure(1e-3)
     best, trace = at.search(measure)
-    assert best["encode_variant"] == 4 and best["lb_window"] == 8 and best["decode_grid_mult"] == 2.0
+    assert best["encode_variant"] == 7 and best["lb_window"] == 8 and best["decode_grid_mult"] == 2.0
     assert calls[0] == at.DEFAULT and len(trace) == len(calls)
 
 
@@ -33,13 +33,13 @@ def test_search_drops_a_winner_that_does_not_confirm():
         if cfg["lb_window"] == 16:
             seen["w16"] += 1
             e = 0.9 if seen["w16"] == 1 else 1.0      # fast only on its first timing
-        if cfg["encode_variant"] == 4:
+        if cfg["encode_variant"] == 7:
             e *= 0.95                                   # fast every time
         d = 0.8 if cfg["decode_grid_mult"] == 2.0 and cfg["decoder"] == 0 else 1.0
         return e * 1e-3, d * 1e-3, "same"
 
     best, trace = at.search(measure)
-    assert best["lb_window"] == at.DEFAULT["lb_window"] and best["encode_variant"] == 4
+    assert best["lb_window"] == at.DEFAULT["lb_window"] and best["encode_variant"] == 7
     assert best["decode_grid_mult"] == 2.0
     assert seen["w16"] > 1   # it was timed again
 

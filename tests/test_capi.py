@@ -75,7 +75,7 @@ def test_sizes_and_messages():
     for st in (0, -1, -2, -3, -4, -5, -6, -7):
         assert lib.sleek_gpu_strerror(st)
     assert lib.sleek_gpu_set_tuning(-1, 0, 0) == _lib.EINVAL
-    assert lib.sleek_gpu_set_tuning(7, 0, 0) == _lib.EINVAL
+    assert lib.sleek_gpu_set_tuning(8, 0, 0) == _lib.EINVAL
     assert lib.sleek_gpu_set_tuning(3, 0, 0) == _lib.EINVAL   # retired variant
     assert lib.sleek_gpu_set_help_after(-1) == _lib.EINVAL
     assert lib.sleek_gpu_set_help_after(10_000_001) == _lib.EINVAL

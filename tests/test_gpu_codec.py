@@ -113,13 +113,14 @@ def help_events():
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])
-@pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6, 7])
 def test_encoder_variant_invariance(gpu, dt, variant):
     """Every encoder kernel (sleek_gpu_set_tuning's encode_variant) writes the oracle's bytes,
     at the automatic grid and at grids of 1, 3, 5 and 8 workgroups that make each one walk
-    many chunks (the ticket encoder then runs fewer ticket sets than eight, one per
-    workgroup).  The ticket encoder (0 = 6) hands every chunk to a running workgroup, so
-    its look-backs never need to size a missing chunk themselves: no help event."""
+    many chunks (the ticket encoders then run 1, 2, 4 or 8 ticket sets, never more
+    than the workgroups).  The ticket encoders (0 = 6, and 7) hand every chunk to a running
+    workgroup, so their look-backs never need to size a missing chunk themselves: no help
+    event."""
     say(f"encoder variant={variant} {np.dtype(dt).name}")
     from sleek_amd import _lib
     from sleek_amd.datagen import exact_field, inject_specials
@@ -137,13 +138,13 @@ def test_encoder_variant_invariance(gpu, dt, variant):
                 assert s.tobytes() == ref.tobytes(), (grid, eb)
                 od, rc = po.decompress(ref, dt, eb)
                 assert rc == 0 and d.tobytes() == od.tobytes()
-            if variant in (0, 6):
+            if variant in (0, 6, 7):
                 assert help_events() == h0, f"ticket encoder needed look-back help at grid {grid}"
     finally:
         lib.sleek_gpu_set_tuning(0, 0, 0)
 
 
-@pytest.mark.parametrize("variant", [0, 1, 4, 5])
+@pytest.mark.parametrize("variant", [0, 1, 4, 5, 7])
 def test_lookback_help_always_on(gpu, variant):
     """With the help threshold at 0 every look-back that does not settle at its first
     poll sizes the missing chunks and groups from the input itself (wave_chunk_size,
@@ -198,7 +199,7 @@ def test_decoder_variant_invariance(gpu, variant):
         lib.sleek_gpu_set_tuning(0, 0, 0)
 
 
-@pytest.mark.parametrize("variant", [1, 4, 5, 6])
+@pytest.mark.parametrize("variant", [1, 4, 5, 6, 7])
 def test_lookback_geometry_invariance(gpu, variant):
     """Look-back group sizes 8..64 and initial windows 1..64 (sleek_gpu_set_lookback)
     change no byte, at the automatic grid and at a 3-workgroup grid."""

@@ -26,6 +26,7 @@ extern "C" {
 #define SLEEK_ECAPACITY (-5)   /* decoded size exceeds the output capacity */
 #define SLEEK_EWORKSPACE (-6)  /* workspace too small */
 #define SLEEK_EALIGN (-7)      /* pointer not aligned as required */
+#define SLEEK_ETOOLARGE (-8)   /* input above sleek_gpu_compress_noalloc's maximum */
 
 #define SLEEK_CHUNK_BYTES 16384
 
@@ -140,6 +141,20 @@ int sleek_gpu_set_lookback(int group_chunks, int window_groups);
  * bytes; applies to later calls. */
 int sleek_gpu_set_decoder(int variant);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  sleek_gpu_compress_async with the library's static device
+ * scratch as the workspace: it calls no hipMalloc, hipFree or allocating
+ * routine.  Inputs up to 4 GiB (4,294,967,296 bytes); larger ones give
+ * SLEEK_ETOOLARGE.  The scratch (2,625,664 bytes) is held by each device that
+ * the library runs on until the process ends, and every call on a device
+ * shares it: calls that may overlap on one device (other streams, other
+ * host threads) race and must not be issued.  The call clears what must
+ * start at zero with a kernel, so it can be captured.  Same bytes as
+ * sleek_gpu_compress_async; the encoder reports no stream errors. */
+int sleek_gpu_compress_noalloc(int elem_bytes, int lossy, int eb_e, int thr_e, unsigned long long offs_e,
+                               const void* d_input, long long insize, unsigned char* d_encoded,
+                               long long* d_encsize, int blocks, void* stream);
 /* Synchronises `stream`, then returns SLEEK_OK, SLEEK_ECORRUPT or
  * SLEEK_ECAPACITY from the workspace's status word. */
 int sleek_gpu_workspace_status(const void* d_workspace, void* stream);

@@ -18,6 +18,7 @@ EEB = -4
 ECAPACITY = -5
 EWORKSPACE = -6
 EALIGN = -7
+ETOOLARGE = -8
 
 c_ll = ctypes.c_longlong
 c_p = ctypes.c_void_p
@@ -51,6 +52,7 @@ SIGNATURES = [
     ("sleek_gpu_set_help_after", c_i, [c_i]),
     ("sleek_gpu_help_events", c_ll, []),
     ("sleek_gpu_workspace_status", c_i, [c_p, c_p]),
+    ("sleek_gpu_compress_noalloc", c_i, [c_i, c_i, c_i, c_i, c_ull, c_p, c_ll, c_p, c_p, c_i, c_p]),
     ("sleek_gpu_batch_plan", c_i, [c_p, c_i, c_i, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)]),
     ("sleek_gpu_batch_compress_async", c_i, [c_p, c_i, c_ll, c_p, c_ll, c_i, c_p]),
     ("sleek_gpu_batch_decompress_async", c_i, [c_p, c_i, c_ll, c_p, c_ll, c_i, c_p]),

@@ -284,6 +284,7 @@ const char* sleek_gpu_strerror(int status)
     case SLEEK_ECAPACITY: return "decoded size exceeds the output capacity";
     case SLEEK_EWORKSPACE: return "workspace too small";
     case SLEEK_EALIGN: return "pointer not aligned (16 B for arrays, 8 B for streams)";
+    case SLEEK_ETOOLARGE: return "input larger than the no-allocation entry point's scratch allows (4 GiB)";
     default: return "unknown status";
   }
 }
@@ -411,6 +412,21 @@ int sleek_gpu_decompress_async(int elem_bytes, int lossy, int thr_e, unsigned lo
 {
   return decompress_impl(elem_bytes, lossy != 0, thr_e, offs_e, d_encoded, enclen, d_decoded, capacity, d_decsize,
                          d_workspace, workspace_bytes, blocks, (hipStream_t)stream);
+}
+
+int sleek_gpu_compress_noalloc(int elem_bytes, int lossy, int eb_e, int thr_e, unsigned long long offs_e,
+                               const void* d_input, long long insize, unsigned char* d_encoded, long long* d_encsize,
+                               int blocks, void* stream)
+{
+  if (insize < 0) return SLEEK_EINVAL;
+  if (insize > sleek::kNoAllocMaxBytes) return SLEEK_ETOOLARGE;
+  static_assert(sleek::kNoAllocScratchBytes % 16 == 0, "scratch size");
+  void* const ws = sleek::noalloc_scratch();
+  if (!ws) return hip_fail(hipErrorInvalidSymbol, "no-allocation scratch");
+  const long long need = ws_bytes_for(insize);
+  if (need > sleek::kNoAllocScratchBytes) return SLEEK_ETOOLARGE;   // cannot happen below the maximum
+  return compress_impl(elem_bytes, lossy != 0, eb_e, thr_e, offs_e, d_input, insize, d_encoded, d_encsize, ws, need,
+                       blocks, (hipStream_t)stream);
 }
 
 int sleek_gpu_workspace_status(const void* d_workspace, void* stream)

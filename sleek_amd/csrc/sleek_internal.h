@@ -97,6 +97,16 @@ constexpr unsigned int kHelpNapsDefault = 100 * kHelpNapsPerUs;
 // HIP error); kernels add one per missing word they size themselves.
 unsigned long long* help_counter();
 
+// The no-allocation compression (sleek_gpu_compress_noalloc): inputs up to
+// kNoAllocMaxBytes (4 GiB, 262,144 chunks), whose workspace fits the
+// library's static device scratch of kNoAllocScratchBytes.
+constexpr long long kNoAllocMaxBytes = 4ll << 30;
+constexpr long long kNoAllocChunks = kNoAllocMaxBytes / 16384;
+constexpr long long kNoAllocScratchBytes =
+    (16 + kNoAllocChunks + 2 * (kNoAllocChunks >> kGroupShiftMin) + kTicketWords) * 8;
+// The current device's address of that scratch (null on a HIP error).
+void* noalloc_scratch();
+
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream);
 // Clears `words` 8-byte words with vector stores (no memset node when the
 // caller captures a graph).  p is 8-byte aligned.

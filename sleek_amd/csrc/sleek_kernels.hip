@@ -26,6 +26,30 @@ namespace sleek {
 // get its device address (help_counter) as an argument.
 __device__ unsigned long long help_events;
 
+// Scratch of sleek_gpu_compress_noalloc: the workspace of one compression
+// of up to kNoAllocMaxBytes input bytes, in device memory that the code
+// object brings (each device holds its own copy for as long as the library
+// is loaded), so the call allocates nothing.  Host code never passes the
+// array's name: it uses the device address noalloc_scratch() looked up for
+// the current device.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+void* noalloc_scratch()
+{
+  static std::mutex mu;
+  static std::vector<void*> addr;   // per device
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return nullptr;
+    addr[dev] = p;
+  }
+  return addr[dev];
+}
+
 unsigned long long* help_counter()
 {
   static std::mutex mu;

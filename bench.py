@@ -386,15 +386,29 @@ def bench_serve(args, dev):
     return out
 
 
-def measured_traffic(kernel, shape_scale):
-    """HBM bytes per dispatch of the dominant kernel, as rocprofv3 measured them
-    for this workload (FETCH_SIZE / WRITE_SIZE, calibrated; profiles/), or None."""
-    p = os.path.join(ROOT, "profiles", "traffic_512cubed_f32.json")
-    if shape_scale != 1.0 or not kernel.startswith("encode (lossless)") or not os.path.exists(p):
-        return None, None
-    t = json.load(open(p))
-    return t["fetch_bytes"] + t["write_bytes"], {"fetch_bytes": t["fetch_bytes"], "write_bytes": t["write_bytes"],
-                                                  "kernel": t["kernel"], "source": "profiles/traffic_512cubed_f32.json"}
+def profiled(workload, mode):
+    """The rocprofv3 figures of one mode of a benchmark field (profiles/codec_kernels.json:
+    kernel trace durations and calibrated FETCH_SIZE / WRITE_SIZE per dispatch, for the
+    encoder and the decoder), or None."""
+    p = os.path.join(ROOT, "profiles", "codec_kernels.json")
+    if not os.path.exists(p):
+        return None
+    return json.load(open(p)).get("workloads", {}).get(workload, {}).get("modes", {}).get(mode)
+
+
+def add_traffic(per, workload):
+    """HBM bytes per dispatch (rocprofv3, calibrated) and the kernels' own durations next to
+    each mode's event-timed calls."""
+    for row in per:
+        prof = profiled(workload, row["mode"])
+        if prof is None:
+            continue
+        for side in ("encode", "decode"):
+            k = prof[side]
+            row[f"{side}_traffic_bytes"] = k["fetch_bytes"] + k["write_bytes"]
+            row[f"{side}_fetch_bytes"] = k["fetch_bytes"]
+            row[f"{side}_write_bytes"] = k["write_bytes"]
+            row[f"{side}_kernel_ms_rocprof"] = round(k["avg_us"] / 1e3, 4)
 
 
 def help_events():
@@ -564,6 +578,8 @@ def bench_f64(args, dev):
     ebs = [None] + [float(np.float64(r) * np.float64(e)) for e in rels]
     labels = ["lossless"] + [f"eb={e:g}*range" for e in rels]
     per, _, t_total, _ = run_modes(x, ebs, labels, max(3, args.steps // 2), 1)
+    if args.scale == 1.0:
+        add_traffic(per, "f64")
     nbytes = x.numel() * 8
     out = {"workload": "384^3 float64, 12 sinusoids + lognormal(-10, 2) noise, 0.01% specials",
            "bytes": nbytes, "per_mode": per, "memcpy_d2d": memcpy_d2d(x),
@@ -626,7 +642,12 @@ def main():
         cands.append((p["decompress_ms"], f"decode ({p['mode']})", alg))
     tms, kname, alg = max(cands)
     achieved = alg / (tms / 1e3) / 1e9
-    traffic, traffic_detail = measured_traffic(kname, args.scale)
+    if args.scale == 1.0:
+        add_traffic(per, "f32")
+    dmode = kname.split("(", 1)[1].rstrip(")")
+    dside = kname.split(" ", 1)[0]
+    prof = profiled("f32", dmode) if args.scale == 1.0 else None
+    kprof = prof[dside] if prof else None
 
     # hipMemcpy device-to-device comparator of the same bytes
     mc = memcpy_d2d(x)
@@ -651,8 +672,16 @@ def main():
                                "per_mode; batched vs plain in batched; serve p50/p99 in serve",
                    "shape": [int(round(512 * args.scale))] * 3, "bytes": nbytes},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9,
-                     "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": traffic,
-                     "traffic_detail": traffic_detail, "alg_bytes_per_launch": alg, "avg_launch_ms": round(tms, 4)},
+                     "unit": "GB/s", "frac": round(achieved * 1e9 / HBM_PEAK, 4),
+                     "traffic": (kprof["fetch_bytes"] + kprof["write_bytes"]) if kprof else None,
+                     "traffic_detail": ({"fetch_bytes": kprof["fetch_bytes"], "write_bytes": kprof["write_bytes"],
+                                         "kernel": kprof["kernel"], "source": "profiles/codec_kernels.json"}
+                                        if kprof else None),
+                     "alg_bytes_per_launch": alg, "avg_launch_ms": round(tms, 4),
+                     "timing": "avg_launch_ms: median of the event-timed calls in this run (HIP events on the launch "
+                               "stream around the whole call: the workspace clear or offset scan plus the kernel)",
+                     "kernel_ms_rocprof": round(kprof["avg_us"] / 1e3, 4) if kprof else None,
+                     "frac_kernel_rocprof": (round(alg / (kprof["avg_us"] * 1e-6) / HBM_PEAK, 4) if kprof else None)},
         "per_mode": per,
         "tuning": tuning,
         "memcpy_d2d": mc,

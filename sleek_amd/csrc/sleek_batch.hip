@@ -52,8 +52,8 @@ struct EncTile {
 // Look-back settings shared by every tile of a batched launch.
 struct LookCfg {
   int lb_shift, lb_window;
-  unsigned long long* helps;
-  unsigned int help_naps;
+  unsigned long long* help;
+  int ticket_shift;   // tile tickets: 2^ticket_shift sets (ticket_shift_for(grid))
 };
 
 __device__ __forceinline__ EncTile enc_tile(const BatchItem* __restrict__ items, const uint32_t* __restrict__ tile_map,
@@ -72,8 +72,7 @@ __device__ __forceinline__ EncTile enc_tile(const BatchItem* __restrict__ items,
   e.a.lb_shift = lc.lb_shift;
   e.a.lb_window = lc.lb_window;
   e.a.tickets = nullptr;
-  e.a.helps = lc.helps;
-  e.a.help_naps = lc.help_naps;
+  e.a.help = lc.help;
   e.nchunks = (e.a.insize + kChunk - 1) / kChunk;
   e.c = t - it->tile0;
   e.kind = item_kind(it);
@@ -210,19 +209,18 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const int h = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
   long long t = blockIdx.x;
-  const Tickets tk = ticket_setup(tickets, tiles);
   if (TICKET) {
     if (h == 0) {
-      const long long m = ticket_chunk(tk, ticket_request(tk), tiles);
+      const long long m = ticket_first(tickets, lc.ticket_shift, tiles);
       if (lane == 0) xfirst = m;
     }
     __syncthreads();
     t = uni(xfirst);
-    if (t < 0) return;   // uniform: every tile is taken
+    if (t < 0) return;   // uniform: the home set is spent, the others take the tiles
   }
   if (t >= tiles) return;
   uint32_t tq = 0;
-  if (TICKET && h == 0) tq = ticket_request(tk);
+  if (TICKET && h == 0) tq = ticket_request(tickets, lc.ticket_shift, tq);
   EncTile ec = enc_tile(items, tile_map, t, ws, lc);
   uint2 cur[kHalfSub];
   uint32_t lh[4];
@@ -233,7 +231,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     for (int i = 0; i < 4; i++) xl[0][4 * h + i] = lh[i];
   }
   if (TICKET && h == 0) {
-    const long long m = ticket_chunk(tk, tq, tiles);
+    const long long m = ticket_chunk(tickets, lc.ticket_shift, tq, tiles);
     if (lane == 0) xnext = m;
   }
   __syncthreads();
@@ -242,7 +240,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   int par = 1;
   long long n = TICKET ? uni(xnext) : t + gridDim.x;
   while (TICKET ? n >= 0 : n < tiles) {
-    if (TICKET && h == 0) tq = ticket_request(tk);
+    if (TICKET && h == 0) tq = ticket_request(tickets, lc.ticket_shift, tq);
     const EncTile en = enc_tile(items, tile_map, n, ws, lc);
     uint2 nxt[kHalfSub];
     load_half<true>(en.a, en.c, h, nxt);
@@ -261,7 +259,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       __builtin_amdgcn_s_setprio(0);
       enc_settle(ec, sc.csize, &xexcl);
       if (TICKET) {
-        const long long m = ticket_chunk(tk, tq, tiles);
+        const long long m = ticket_chunk(tickets, lc.ticket_shift, tq, tiles);
         if (lane == 0) xnext = m;
       }
     }
@@ -365,10 +363,9 @@ int batch_resident_grid(bool decompress, bool pair)
 }
 
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, unsigned int help_naps,
-                        hipStream_t stream)
+                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, hipStream_t stream)
 {
-  const LookCfg lc{lb_shift, lb_window, help_counter(), help_naps};
+  LookCfg lc{lb_shift, lb_window, help_state(), 0};
   if (n <= 0) return hipSuccess;
   uint32_t* const map = (uint32_t*)(ws + batch_map_word(n));
   const long long res = batch_resident_grid(decompress, pair);
@@ -379,6 +376,7 @@ hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long lon
   if (!decompress && res > 0 && g > res) g = res;   // the encoder's look-back: no more than the resident set
   if (g <= 0) g = 1024;
   if (g > tiles) g = tiles > 0 ? tiles : 1;
+  lc.ticket_shift = ticket_shift_for(g);
   const bool tk = !decompress && pair && tiles > 0;
   if (tk && !tickets) return hipErrorInvalidValue;
   hipLaunchKernelGGL(batch_prep_kernel, dim3(n), dim3(256), 0, stream, items, map, ws, decompress ? 0 : 1,

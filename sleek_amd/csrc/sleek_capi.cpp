@@ -51,7 +51,6 @@ std::atomic<int> g_decode_grid{0};
 std::atomic<int> g_decode_variant{sleek::kDecodeScan};
 std::atomic<int> g_lb_shift{5};    // look-back groups of 32 chunks
 std::atomic<int> g_lb_window{0};   // initial look-back window, in groups (0: the encoder's default)
-std::atomic<unsigned int> g_help_naps{sleek::kHelpNapsDefault};   // look-back help threshold (naps)
 
 // Per-device scratch of the synchronous entry points (grow-only).
 struct DeviceScratch {
@@ -129,9 +128,9 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   a.lb_shift = g_lb_shift;
   a.lb_window = g_lb_window > 0 ? (int)g_lb_window : sleek::default_window(sleek::resolve_variant(g_encode_variant, elem_bytes));
   a.tickets = (unsigned int*)((uint64_t*)ws + ws_words_for(insize));
-  a.helps = sleek::help_counter();
-  a.help_naps = g_help_naps;
+  a.help = sleek::help_state();
   const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
+  a.ticket_shift = sleek::ticket_shift_for(grid);
   e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_variant, stream);
   if (e != hipSuccess) return hip_fail(e, "encode launch");
   return SLEEK_OK;
@@ -375,13 +374,13 @@ int sleek_gpu_set_lookback(int group_chunks, int window_groups)
 int sleek_gpu_set_help_after(int microseconds)
 {
   if (microseconds < 0 || microseconds > 10000000) return SLEEK_EINVAL;
-  g_help_naps = (unsigned int)microseconds * sleek::kHelpNapsPerUs;
-  return SLEEK_OK;
+  const hipError_t e = sleek::set_help_naps((unsigned long long)microseconds * sleek::kHelpNapsPerUs);
+  return e == hipSuccess ? SLEEK_OK : hip_fail(e, "help threshold");
 }
 
 long long sleek_gpu_help_events(void)
 {
-  unsigned long long* p = sleek::help_counter();
+  unsigned long long* p = sleek::help_state();
   if (!p) return hip_fail(hipErrorInvalidSymbol, "help counter");
   hipError_t e = hipDeviceSynchronize();
   if (e != hipSuccess) return hip_fail(e, "hipDeviceSynchronize");
@@ -502,7 +501,7 @@ static int batch_async(bool decompress, const sleek_gpu_batch_item* d_items, int
   uint32_t* const tickets = (uint32_t*)((uint64_t*)ws + even_end - sleek::kTicketWords);
   const int window = g_lb_window > 0 ? (int)g_lb_window : sleek::default_window(pair ? sleek::kEncodePairTicket : rv);
   const hipError_t e = sleek::batch_launch(decompress, (const sleek::BatchItem*)d_items, n, tiles, (uint64_t*)ws, grid,
-                                           g_lb_shift, window, pair, tickets, g_help_naps, (hipStream_t)stream);
+                                           g_lb_shift, window, pair, tickets, (hipStream_t)stream);
   return e == hipSuccess ? SLEEK_OK : hip_fail(e, "batch launch");
 }
 

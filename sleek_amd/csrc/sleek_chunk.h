@@ -122,10 +122,17 @@ __device__ uint64_t wave_group_size(const EncodeArgs& a, const ScanState& s, lon
 }
 
 // One help event (a look-back sized a missing word itself) on the device's
-// counter, if there is one.
-__device__ __forceinline__ void count_help(unsigned long long* helps)
+// counter (help[0]), if there is one.
+__device__ __forceinline__ void count_help(unsigned long long* help)
 {
-  if (helps) __hip_atomic_fetch_add(helps, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (help) __hip_atomic_fetch_add(help, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The look-back's help threshold in naps (help[1]; the default without a
+// help state), read only when a wait has gone on a while.
+__device__ __forceinline__ uint32_t help_naps(const unsigned long long* help)
+{
+  return help ? (uint32_t)help[1] : kHelpNapsDefault;
 }
 
 // Two-level decoupled look-back, run by one whole wave; returns the sum of
@@ -138,14 +145,14 @@ __device__ __forceinline__ void count_help(unsigned long long* helps)
 // Polls back off exponentially: the words live at device scope, and many
 // waiting waves polling at full rate would compete with the data traffic.
 // Workgroups take chunks in launch or ticket order, so the words waited on
-// are normally published within a few round trips.  After a.help_naps naps
+// are normally published within a few round trips.  After help_naps() naps
 // without progress (about 100 us of the wave's own sleep by default; time
 // the wave spends suspended with its queue does not count) the waiter
 // computes missing words from the input and publishes them, one after
 // another without further waits, until others' words arrive again; so the
 // look-back finishes even when a predecessor's workgroup is not running (an
 // encoder with chunks in a static order whose grid exceeds what is
-// resident).  Each word sized so is counted in *a.helps: the ticket encoders
+// resident).  Each word sized so is counted in a.help[0]: the ticket encoders
 // hand chunks only to running workgroups and should never need it.
 template <typename W, bool LOSSY>
 __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long c, long long nchunks)
@@ -174,14 +181,14 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         have_partial = true;
         progressed = true;
       }
-      if (!progressed && (helping || slept >= a.help_naps)) {
+      if (!progressed && (helping || slept >= help_naps(a.help))) {
         const uint64_t miss = __ballot(lane < q && (f >> 62) == 0);
         if (miss) {
           const long long jc = c - __ffsll((unsigned long long)miss);
           const uint32_t sz = wave_chunk_size<W, LOSSY>(a, jc);
           if (lane == 0) {
             flag_max(s.chunk + jc, kAgg | sz);
-            count_help(a.helps);
+            count_help(a.help);
           }
         }
         helping = helped = true;
@@ -213,14 +220,14 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         gwin = min(2 * gwin, 64);
         progressed = true;
       }
-      if (!progressed && (helping || slept >= a.help_naps)) {
+      if (!progressed && (helping || slept >= help_naps(a.help))) {
         const uint64_t miss = __ballot((fg >> 62) == 0);
         if (miss) {
           const long long gm = gend - __ffsll((unsigned long long)miss);
           const uint64_t sz = wave_group_size<W, LOSSY>(a, s, gm, nchunks);
           if (lane == 0) {
             flag_max(s.group + gm, (gm == 0 ? kInc : kAgg) | sz);
-            count_help(a.helps);
+            count_help(a.help);
           }
         }
         helping = helped = true;
@@ -432,18 +439,14 @@ __device__ __forceinline__ void finish_chunk(const EncodeArgs& a, const ScanStat
 constexpr int kHalfSub = kSub / 2;
 
 // ---- chunk tickets (encode_pair_kernel<.., TICKET>) ----
-// S = 2^shift counters, the largest power of two <= min(kTicketSets, grid),
-// so that every set has a workgroup of its own (its home set, blockIdx.x mod
-// S); ticket t of set x is chunk S * t + x.  A workgroup takes every chunk,
-// its first included, from the counters, so a chunk is only ever held by a
-// workgroup that runs.  The set geometry and the home set's ticket count are
-// computed once per kernel (shifts and masks only: no division).
-struct Tickets {
-  uint32_t* tk;     // the counters, kTicketStride dwords apart
-  int shift;        // S = 1 << shift sets
-  int home;         // blockIdx.x mod S
-  uint32_t limit;   // tickets of the home set
-};
+// S = 2^shift counters, the largest power of two <= min(kTicketSets, grid)
+// (ticket_shift_for, computed by the host), so that every set has a
+// workgroup of its own (its home set, blockIdx.x mod S); ticket t of set x
+// is chunk S * t + x.  A workgroup takes every chunk, its first included,
+// from the counters, so a chunk is only ever held by a workgroup that runs.
+// The home set is recomputed from blockIdx where it is used (a mask) rather
+// than kept live across the encoder's loop, whose registers are all in use.
+__device__ __forceinline__ int ticket_home(int shift) { return (int)(blockIdx.x & ((1u << shift) - 1u)); }
 
 // Tickets of set x: chunks S * t + x below nchunks (nchunks < 2^32).
 __device__ __forceinline__ uint32_t ticket_limit(uint32_t nchunks, int x, int shift)
@@ -451,29 +454,21 @@ __device__ __forceinline__ uint32_t ticket_limit(uint32_t nchunks, int x, int sh
   return nchunks > (uint32_t)x ? ((nchunks - (uint32_t)x - 1u) >> shift) + 1u : 0u;
 }
 
-__device__ __forceinline__ Tickets ticket_setup(uint32_t* tk, long long nchunks)
+// A ticket of the workgroup's home set, in lane 0; the other lanes keep
+// `prev` (the previous ticket, in the register the atomic writes: a value of
+// their own, not undefined, and no instruction that writes or reads the
+// register before the atomic, which would wait for the previous atomic and
+// so for every memory operation in flight).  The result is left in flight:
+// it is read by ticket_chunk.  The address is made opaque so that the
+// compiler does not turn the lane's atomic into a wave-wide one, whose
+// result it would wait for at once.  Callers issue it before loads they wait
+// for early (memory operations complete in issue order as far as those waits
+// are concerned).
+__device__ __forceinline__ uint32_t ticket_request(uint32_t* tk, int shift, uint32_t prev)
 {
-  Tickets k;
-  const unsigned g = gridDim.x;
-  k.tk = tk;
-  k.shift = g >= 8 ? 3 : (g >= 4 ? 2 : (g >= 2 ? 1 : 0));
-  static_assert(kTicketSets == 8, "ticket sets: 8");
-  k.home = (int)(blockIdx.x & ((1u << k.shift) - 1u));
-  k.limit = ticket_limit((uint32_t)nchunks, k.home, k.shift);
-  return k;
-}
-
-// A ticket of the workgroup's home set, in lane 0 (other lanes: 0).  The
-// result is left in flight: it is read by ticket_chunk.  The address is made
-// opaque so that the compiler does not turn the lane's atomic into a
-// wave-wide one, whose result it would wait for at once.  Callers issue it
-// before loads they wait for early (memory operations complete in issue
-// order as far as those waits are concerned).
-__device__ __forceinline__ uint32_t ticket_request(const Tickets& k)
-{
-  uint32_t t = 0;
+  uint32_t t = prev;
   if ((threadIdx.x & 63) == 0) {
-    uintptr_t p = (uintptr_t)(k.tk + k.home * kTicketStride);
+    uintptr_t p = (uintptr_t)(tk + ticket_home(shift) * kTicketStride);
     asm volatile("" : "+v"(p));
     t = __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t*)p, 1u, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
@@ -487,12 +482,13 @@ __device__ __forceinline__ uint32_t ticket_request(const Tickets& k)
 // spent and each ticket below a limit goes to exactly one workgroup: every
 // chunk is taken.  A set whose ticket came back spent is not tried again, so
 // the loop runs at most S - 1 times whatever the reads return.
-__device__ __noinline__ long long ticket_fallback(uint32_t* tk, int shift, int home, long long nchunks)
+__device__ __noinline__ long long ticket_fallback(uint32_t* tk, int shift, long long nchunks)
 {
   typedef __attribute__((address_space(1))) uint32_t gu32;
   gu32* const g = (gu32*)tk;
   const int lane = threadIdx.x & 63;
   const int S = 1 << shift;
+  const int home = ticket_home(shift);
   uint32_t spent = 1u << home;
   while (true) {
     const int y = (home + 1 + lane) & (S - 1);
@@ -511,13 +507,26 @@ __device__ __noinline__ long long ticket_fallback(uint32_t* tk, int shift, int h
   }
 }
 
+// A workgroup's first chunk: a ticket of its home set (uniform), or -1 when
+// that set is already spent, and the workgroup then has nothing to do (the
+// chunks of every other set are taken by their own workgroups and their
+// fallbacks).  No fallback call here: a second call site of the out-of-line
+// fallback costs the encoders' loops register spills.
+__device__ __forceinline__ long long ticket_first(uint32_t* tk, int shift, long long nchunks)
+{
+  const int home = ticket_home(shift);
+  const uint32_t t = uni(ticket_request(tk, shift, 0u));
+  return t < ticket_limit((uint32_t)nchunks, home, shift) ? ((long long)t << shift) + home : -1;
+}
+
 // The chunk of a ticket from ticket_request (uniform), or -1 once every set
 // is spent.
-__device__ __forceinline__ long long ticket_chunk(const Tickets& k, uint32_t t, long long nchunks)
+__device__ __forceinline__ long long ticket_chunk(uint32_t* tk, int shift, uint32_t t, long long nchunks)
 {
+  const int home = ticket_home(shift);
   t = uni(t);
-  if (t < k.limit) return ((long long)t << k.shift) + k.home;
-  return ticket_fallback(k.tk, k.shift, k.home, nchunks);
+  if (t < ticket_limit((uint32_t)nchunks, home, shift)) return ((long long)t << shift) + home;
+  return ticket_fallback(tk, shift, nchunks);
 }
 
 // GLOBAL: the array pointer was read from memory (a batch item): load

@@ -20,10 +20,9 @@ struct EncodeArgs {
   unsigned long long offs;
   int lb_shift;           // look-back groups of 2^lb_shift chunks (3..6)
   int lb_window;          // initial look-back window in groups (1..64)
-  unsigned int* tickets;  // zeroed chunk counters of kEncodePairTicket (kTicketSets, kTicketStride apart)
-  unsigned long long* helps;   // device counter of look-back help events (help_counter())
-  unsigned int help_naps;      // a look-back that has slept this many naps (128 clocks each) without
-                               // progress sizes the missing words itself
+  unsigned int* tickets;  // zeroed chunk counters of the ticket encoders (kTicketSets, kTicketStride apart)
+  int ticket_shift;       // 2^ticket_shift ticket sets are in use: ticket_shift_for(grid)
+  unsigned long long* help;     // device help state (help_state()): [0] help events, [1] help threshold in naps
 };
 
 // Chunk tickets (kEncodePairTicket): every chunk, a workgroup's first
@@ -37,6 +36,11 @@ struct EncodeArgs {
 constexpr int kTicketSets = 8;
 constexpr int kTicketStride = 128;
 constexpr long long kTicketWords = kTicketSets * kTicketStride / 2;   // 8-byte workspace words
+// log2 of the ticket sets a launch of `grid` workgroups uses: the largest
+// power of two up to min(kTicketSets, grid), so that every set has a
+// workgroup of its own
+inline int ticket_shift_for(long long grid) { return grid >= 8 ? 3 : (grid >= 4 ? 2 : (grid >= 2 ? 1 : 0)); }
+static_assert(kTicketSets == 8, "ticket_shift_for assumes eight sets");
 
 struct DecodeArgs {
   const uint8_t* in;      // stream (8-byte aligned)
@@ -93,9 +97,15 @@ inline int default_window(int resolved_variant) { return ticket_variant(resolved
 // missing predecessor itself: about 100 us.
 constexpr unsigned int kHelpNapsPerUs = 19;
 constexpr unsigned int kHelpNapsDefault = 100 * kHelpNapsPerUs;
-// The device address of the calling device's help-event counter (null on a
-// HIP error); kernels add one per missing word they size themselves.
-unsigned long long* help_counter();
+// The device address of the calling device's help state (null on a HIP
+// error): [0] the count of help events (kernels add one per missing word
+// they size themselves), [1] the help threshold in naps (a look-back that has
+// slept this many naps of 128 clocks without progress sizes the missing words
+// itself).
+unsigned long long* help_state();
+// Sets the help threshold (naps) of every device: those whose help state has
+// been looked up now (a synchronous copy each), the others when it is.
+hipError_t set_help_naps(unsigned long long naps);
 
 // The no-allocation compression (sleek_gpu_compress_noalloc): inputs up to
 // kNoAllocMaxBytes (4 GiB, 262,144 chunks), whose workspace fits the
@@ -150,8 +160,7 @@ inline long long batch_map_word(int n) { return kBatchHead + even_words(n); }
 // kTicketWords words); else the one-wave batched encoder, tiles in launch
 // order
 hipError_t batch_launch(bool decompress, const BatchItem* items, int n, long long tiles, uint64_t* ws, int grid,
-                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, unsigned int help_naps,
-                        hipStream_t stream);
+                        int lb_shift, int lb_window, bool pair, uint32_t* tickets, hipStream_t stream);
 hipError_t batch_status_launch(const BatchItem* items, int n, uint64_t* ws, hipStream_t stream);
 int batch_resident_grid(bool decompress, bool pair);
 

@@ -21,10 +21,11 @@
 
 namespace sleek {
 
-// Look-back help events on this device since the library was loaded (each
-// one a missing look-back word that a waiting wave sized itself).  Kernels
-// get its device address (help_counter) as an argument.
-__device__ unsigned long long help_events;
+// Look-back help state of this device: [0] help events since the library
+// was loaded (each one a missing look-back word that a waiting wave sized
+// itself), [1] the help threshold in naps (sleek_gpu_set_help_after).
+// Kernels get its device address (help_state) as an argument.
+__device__ unsigned long long help_words[2] = {0ull, kHelpNapsDefault};
 
 // Scratch of sleek_gpu_compress_noalloc: the workspace of one compression
 // of up to kNoAllocMaxBytes input bytes, in device memory that the code
@@ -50,20 +51,57 @@ void* noalloc_scratch()
   return addr[dev];
 }
 
-unsigned long long* help_counter()
+// The help state's per-device addresses and the threshold a device's state
+// is given when its address is first looked up (kHelpNapsDefault until
+// sleek_gpu_set_help_after).
+static std::mutex help_mu;
+static std::vector<unsigned long long*> help_addr;
+static unsigned long long help_naps_host = kHelpNapsDefault;
+
+static hipError_t write_help_naps(unsigned long long* state, unsigned long long naps)
 {
-  static std::mutex mu;
-  static std::vector<unsigned long long*> addr;   // per device
+  // a synchronous copy: kernels launched after it see the threshold
+  return hipMemcpy(state + 1, &naps, sizeof naps, hipMemcpyHostToDevice);
+}
+
+unsigned long long* help_state()
+{
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  std::lock_guard<std::mutex> lk(mu);
-  if ((int)addr.size() <= dev) addr.resize(dev + 1, nullptr);
-  if (!addr[dev]) {
+  std::lock_guard<std::mutex> lk(help_mu);
+  if ((int)help_addr.size() <= dev) help_addr.resize(dev + 1, nullptr);
+  if (!help_addr[dev]) {
     void* p = nullptr;
-    if (hipGetSymbolAddress(&p, HIP_SYMBOL(help_events)) != hipSuccess) return nullptr;
-    addr[dev] = (unsigned long long*)p;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(help_words)) != hipSuccess || p == nullptr) return nullptr;
+    if (help_naps_host != kHelpNapsDefault && write_help_naps((unsigned long long*)p, help_naps_host) != hipSuccess)
+      return nullptr;
+    help_addr[dev] = (unsigned long long*)p;
   }
-  return addr[dev];
+  return help_addr[dev];
+}
+
+hipError_t set_help_naps(unsigned long long naps)
+{
+  std::lock_guard<std::mutex> lk(help_mu);
+  help_naps_host = naps;
+  int cur = 0;
+  hipError_t e = hipSuccess;
+  bool switched = false;
+  for (int d = 0; d < (int)help_addr.size() && e == hipSuccess; d++) {
+    if (!help_addr[d]) continue;   // set when the device's state is first looked up
+    if (!switched) {
+      e = hipGetDevice(&cur);
+      if (e != hipSuccess) break;
+      switched = true;
+    }
+    e = hipSetDevice(d);
+    if (e == hipSuccess) e = write_help_naps(help_addr[d], naps);
+  }
+  if (switched) {
+    const hipError_t r = hipSetDevice(cur);
+    if (e == hipSuccess) e = r;
+  }
+  return e;
 }
 
 // Encoder: each workgroup walks its chunks (launch order, stride = grid) as
@@ -256,8 +294,8 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(WAVES,
 // live at once (128 VGPRs), two waves per SIMD.
 //
 // TICKET: the wave takes every chunk from the chunk tickets, as the two-wave
-// ticket encoder does: the first before it starts, then the chunk after next,
-// requested at the start of an iteration and read after the look-back.
+// ticket encoder does: the first before it starts, then the chunk after
+// next, requested at the start of an iteration and read after the look-back.
 template <typename W, bool LOSSY, bool TICKET>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void encode_wave2_kernel(EncodeArgs a)
 {
@@ -271,20 +309,19 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     return;
   }
   const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
-  const Tickets tk = ticket_setup(a.tickets, nchunks);
   long long c = blockIdx.x;
-  if (TICKET) c = ticket_chunk(tk, ticket_request(tk), nchunks);   // uniform; -1: every chunk is taken
+  if (TICKET) c = ticket_first(a.tickets, a.ticket_shift, nchunks);   // uniform; -1: the home set is spent
   if (c < 0 || c >= nchunks) return;
   uint32_t tq = 0;
-  if (TICKET) tq = ticket_request(tk);
+  if (TICKET) tq = ticket_request(a.tickets, a.ticket_shift, tq);
   uint2 cur[kSub];
   load_raw(a, c, cur);
   ChunkShape sc = shape_codes<W, LOSSY>(a, c, cur);
   publish_chunk(a, st, c, nchunks, sc.csize);
-  long long n = TICKET ? ticket_chunk(tk, tq, nchunks) : c + gridDim.x;
+  long long n = TICKET ? ticket_chunk(a.tickets, a.ticket_shift, tq, nchunks) : c + gridDim.x;
   while (true) {
     const bool more = TICKET ? n >= 0 : n < nchunks;
-    if (TICKET && more) tq = ticket_request(tk);
+    if (TICKET && more) tq = ticket_request(a.tickets, a.ticket_shift, tq);
     uint2 nxt[kSub];
     if (more) load_raw(a, n, nxt);
     pack_codes<W, LOSSY>(a, c, cur, sc, stage);
@@ -299,7 +336,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
     }
     finish_chunk(a, st, c, nchunks, sc.csize, stage, (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks));
     if (!more) break;
-    const long long nn = TICKET ? ticket_chunk(tk, tq, nchunks) : n + gridDim.x;
+    const long long nn = TICKET ? ticket_chunk(a.tickets, a.ticket_shift, tq, nchunks) : n + gridDim.x;
     c = n;
     n = nn;
     sc = sn;
@@ -343,21 +380,20 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   const ScanState st = scan_state(a.ws, nchunks, a.lb_shift);
   const long long data_base = 8 + 2 * nchunks;
   long long c = blockIdx.x;
-  const Tickets tk = ticket_setup(a.tickets, nchunks);
   if (TICKET) {
     if (h == 0) {
-      const long long m = ticket_chunk(tk, ticket_request(tk), nchunks);
+      const long long m = ticket_first(a.tickets, a.ticket_shift, nchunks);
       if (lane == 0) xfirst = m;
     }
     __syncthreads();
     c = uni(xfirst);
-    if (c < 0) return;   // uniform: every chunk is taken
+    if (c < 0) return;   // uniform: the home set is spent, the others take the chunks
   }
   if (c >= nchunks) return;
   uint2 cur[kHalfSub];
   uint32_t lh[4];
   uint32_t tq = 0;
-  if (TICKET && h == 0) tq = ticket_request(tk);
+  if (TICKET && h == 0) tq = ticket_request(a.tickets, a.ticket_shift, tq);
   load_half(a, c, h, cur);
   shape_half<W, LOSSY>(a, c, h, cur, lh);
   if (lane == 0) {
@@ -365,7 +401,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     for (int i = 0; i < 4; i++) xl[0][4 * h + i] = lh[i];
   }
   if (TICKET && h == 0) {
-    const long long m = ticket_chunk(tk, tq, nchunks);
+    const long long m = ticket_chunk(a.tickets, a.ticket_shift, tq, nchunks);
     if (lane == 0) xnext = m;
   }
   __syncthreads();
@@ -376,7 +412,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   while (true) {
     const bool more = TICKET ? n >= 0 : n < nchunks;
     uint2 nxt[kHalfSub];
-    if (TICKET && h == 0 && more) tq = ticket_request(tk);
+    if (TICKET && h == 0 && more) tq = ticket_request(a.tickets, a.ticket_shift, tq);
     if (more) load_half(a, n, h, nxt);
     pack_half<W, LOSSY>(a, c, h, cur, sc, stage);
     __builtin_amdgcn_s_setprio(2);   // n's size is waited on by others: as in encode_wave2_kernel
@@ -396,7 +432,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       __builtin_amdgcn_s_setprio(0);
       const uint64_t excl = (c == 0) ? 0ull : look_back<W, LOSSY>(a, st, c, nchunks);
       long long m = -1;
-      if (TICKET && more) m = ticket_chunk(tk, tq, nchunks);
+      if (TICKET && more) m = ticket_chunk(a.tickets, a.ticket_shift, tq, nchunks);
       if (lane == 0) {
         if (c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)sc.csize);
         xexcl = excl;

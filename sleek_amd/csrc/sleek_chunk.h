@@ -157,6 +157,9 @@ __device__ __forceinline__ uint32_t help_naps(const unsigned long long* help)
 template <typename W, bool LOSSY>
 __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long c, long long nchunks)
 {
+  // the per-lane word addresses use opaque(lane): recomputed at each poll (a
+  // few instructions) rather than kept live through the encoders' loops,
+  // whose registers are all in use
   const int lane = threadIdx.x & 63;
   const int q = (int)(c & ((1ll << s.gshift) - 1));   // chunks of the group before c
   long long gend = c >> s.gshift;                       // group window: groups gend-gwin .. gend-1
@@ -169,7 +172,7 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
   while (true) {
     bool progressed = false, helped = false;
     if (!have_partial) {
-      const uint64_t f = (lane < q) ? flag_load(s.chunk + (c - 1 - lane)) : kAgg;   // beyond the group: 0
+      const uint64_t f = (lane < q) ? flag_load(s.chunk + (c - 1 - opaque(lane))) : kAgg;   // beyond the group: 0
       const uint64_t inc = __ballot((f >> 62) == 2);
       const uint64_t rdy = __ballot((f >> 62) != 0);
       if (inc) {
@@ -196,7 +199,7 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
     } else {
       // a group's word, or its completion count once every member has
       // reported (the count's bytes are then the group's aggregate)
-      const long long gj = gend - 1 - lane;
+      const long long gj = gend - 1 - opaque(lane);
       uint64_t fg = kAgg;                      // lanes beyond the window: 0
       if (lane < gwin) {
         if (gj < 0) {
@@ -489,11 +492,11 @@ __device__ __noinline__ long long ticket_fallback(uint32_t* tk, int shift, long 
   const int lane = threadIdx.x & 63;
   const int S = 1 << shift;
   const int home = ticket_home(shift);
-  uint32_t spent = 1u << home;
+  uint64_t spent = 1ull << home;
   while (true) {
     const int y = (home + 1 + lane) & (S - 1);
     bool open = false;
-    if (lane < S - 1 && !((spent >> y) & 1u))
+    if (lane < S - 1 && !((spent >> y) & 1ull))
       open = __hip_atomic_load(g + y * kTicketStride, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <
              ticket_limit((uint32_t)nchunks, y, shift);
     const uint64_t m = __ballot(open);
@@ -503,7 +506,7 @@ __device__ __noinline__ long long ticket_fallback(uint32_t* tk, int shift, long 
     if (lane == 0) u = __hip_atomic_fetch_add(g + z * kTicketStride, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     u = uni(u);
     if (u < ticket_limit((uint32_t)nchunks, z, shift)) return ((long long)u << shift) + z;
-    spent |= 1u << z;
+    spent |= 1ull << z;
   }
 }
 
@@ -534,7 +537,9 @@ __device__ __forceinline__ long long ticket_chunk(uint32_t* tk, int shift, uint3
 template <bool GLOBAL = false>
 __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int h, uint2 raw[kHalfSub])
 {
-  const int lane = threadIdx.x & 63;
+  // opaque: the lane's offset is recomputed here rather than kept live (or
+  // spilled: a reload here would wait for every memory operation in flight)
+  const int lane = opaque(threadIdx.x & 63);
   const long long base = c * kChunk;
   const int osize = (int)min((long long)kChunk, a.insize - base);
   const uint2* const p = (const uint2*)(a.in + base + h * kHalfSub * kSubBytes) + lane;

@@ -367,6 +367,13 @@ __device__ __forceinline__ void reissue(uint4& q)
   asm volatile("" : "+v"(q.x), "+v"(q.y), "+v"(q.z), "+v"(q.w));
 }
 __device__ __forceinline__ void reissue(uint32_t& x) { asm volatile("" : "+v"(x)); }
+// v, recomputed where it is used: values derived from it there are not
+// hoisted out of the enclosing loop (and kept live, or spilled) by the compiler.
+__device__ __forceinline__ int opaque(int v)
+{
+  asm volatile("" : "+v"(v));
+  return v;
+}
 __device__ __forceinline__ void reissue_pair(uint2& q) { asm volatile("" : "+v"(q.x), "+v"(q.y)); }
 __device__ __forceinline__ void reissue(uint64_t& x)
 {

@@ -28,19 +28,26 @@ struct EncodeArgs {
 // Chunk tickets (kEncodePairTicket): every chunk, a workgroup's first
 // included, comes from one of S counters, S the largest power of two up to
 // min(kTicketSets, grid) (home set x = blockIdx.x mod S): ticket t of set x
-// is chunk S * t + x.  A chunk is
-// therefore only ever held by a running workgroup, and every set has one, so
-// the look-back never waits on a workgroup that has not started.  The
-// counters sit kTicketStride dwords (512 bytes) apart, after the look-back
-// words of the workspace.
-constexpr int kTicketSets = 8;
-constexpr int kTicketStride = 128;
+// is chunk S * t + x.  A chunk is therefore only ever held by a running
+// workgroup, and every set has one, so the look-back never waits on a
+// workgroup that has not started.  Many sets keep the counters' queues
+// short: each takes about grid / S requests per chunk time, and all of them
+// at once when the launch starts.  The counters sit kTicketStride dwords (a
+// 128-byte line) apart, after the look-back words of the workspace.
+constexpr int kTicketSets = 64;
+constexpr int kTicketStride = 32;
 constexpr long long kTicketWords = kTicketSets * kTicketStride / 2;   // 8-byte workspace words
+static_assert(kTicketSets <= 64 && (kTicketSets & (kTicketSets - 1)) == 0,
+              "ticket sets: a power of two, at most one per lane of the fallback's wave");
 // log2 of the ticket sets a launch of `grid` workgroups uses: the largest
 // power of two up to min(kTicketSets, grid), so that every set has a
 // workgroup of its own
-inline int ticket_shift_for(long long grid) { return grid >= 8 ? 3 : (grid >= 4 ? 2 : (grid >= 2 ? 1 : 0)); }
-static_assert(kTicketSets == 8, "ticket_shift_for assumes eight sets");
+inline int ticket_shift_for(long long grid)
+{
+  int s = 0;
+  while ((2ll << s) <= grid && (2 << s) <= kTicketSets) s++;
+  return s;
+}
 
 struct DecodeArgs {
   const uint8_t* in;      // stream (8-byte aligned)

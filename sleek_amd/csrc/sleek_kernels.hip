@@ -420,7 +420,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       shape_half<W, LOSSY>(a, n, h, nxt, lh);
       if (lane == 0) {
 #pragma unroll
-        for (int i = 0; i < 4; i++) xl[par][4 * h + i] = lh[i];
+        for (int i = 0; i < 4; i++) xl[par][4 * opaque(h) + i] = lh[i];
       }
     }
     if (h == 1) __builtin_amdgcn_s_setprio(0);   // wave 0 keeps it until n's size is published

@@ -116,9 +116,9 @@ def help_events():
 @pytest.mark.parametrize("variant", [0, 1, 2, 4, 5, 6, 7])
 def test_encoder_variant_invariance(gpu, dt, variant):
     """Every encoder kernel (sleek_gpu_set_tuning's encode_variant) writes the oracle's bytes,
-    at the automatic grid and at grids of 1, 3, 5 and 8 workgroups that make each one walk
-    many chunks (the ticket encoders then run 1, 2, 4 or 8 ticket sets, never more
-    than the workgroups).  The ticket encoders (0 = 6, and 7) hand every chunk to a running
+    at the automatic grid and at grids of 1, 3, 5, 8 and 37 workgroups that make each one
+    walk many chunks (the ticket encoders then run 1, 2, 4, 8 or 32 ticket sets, never more
+    than the workgroups; at the automatic grid, 64 sets of one or two tickets each).  The ticket encoders (0 = 6, and 7) hand every chunk to a running
     workgroup, so their look-backs never need to size a missing chunk themselves: no help
     event."""
     say(f"encoder variant={variant} {np.dtype(dt).name}")
@@ -129,7 +129,7 @@ def test_encoder_variant_invariance(gpu, dt, variant):
     a = exact_field(400_003 if dt == np.float32 else 200_003, 5, dt)
     inject_specials(a, 1e-3, 11)
     try:
-        for grid in (0, 1, 3, 5, 8):
+        for grid in (0, 1, 3, 5, 8, 37):
             _lib.check(lib.sleek_gpu_set_tuning(variant, grid, 0), "tuning")
             h0 = help_events()
             for eb in (None, 1e-2, 1e-5):

@@ -151,7 +151,7 @@ def test_batch_plan_layout(decompress):
         assert it["ws_word"] == word and it["ws_word"] % 2 == 0
         t += ch
         word += it["ws_words"]
-    assert tiles == t and wsb == (word + 512) * 8   # then 4 KiB of tile-ticket counters
+    assert tiles == t and wsb == (word + 1024) * 8   # then 8 KiB of tile-ticket counters (64, 128 B apart)
 
 
 def test_batch_plan_rejects_bad_items():

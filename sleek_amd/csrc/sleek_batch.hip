@@ -299,7 +299,7 @@ __device__ __forceinline__ DecodeArgs decode_args(const BatchItem* it, uint64_t*
 __global__ __launch_bounds__(kScanThreads) void batch_offsets_kernel(const BatchItem* items, uint64_t* ws)
 {
   __shared__ long long wsum[kScanThreads / 64];
-  scan_offsets(decode_args(items + blockIdx.x, ws), wsum);
+  scan_offsets(decode_args(items + blockIdx.x, ws), wsum, 0, 1);
 }
 
 // Decoder: workgroups stride over the tiles, one tile at a time; tiles

@@ -697,37 +697,66 @@ __device__ __forceinline__ bool header_ok(const DecodeArgs& a, long long hdr, lo
 }
 
 // Exclusive prefix sums of a stream's uint16 size table into the workspace
-// (one int64 per chunk plus the total), by one workgroup of kScanThreads
-// threads: each thread takes 32 consecutive entries (8-byte loads), so one
-// pass covers 32768 chunks (512 MiB of f32 input) and larger tables loop.
-// Also writes *decsize; a bad header is reported in the status word.
+// (one int64 per chunk plus the total), by `parts` workgroups of
+// kScanThreads threads that need nothing from one another: the table's
+// rounds of kScanRound entries are split into parts of whole rounds, and
+// part p first sums every entry before its rounds itself (table reads that
+// hit L2) and then scans its rounds from that carry.  Part 0 also writes
+// *decsize, clears the status word and reports a bad header in it.
 constexpr int kScanThreads = 1024;
-constexpr int kScanRows = 8;                                           // rows of 256 entries per wave and round
+constexpr int kScanRows = 2;                                           // rows of 256 entries per wave and round
 constexpr long long kScanBlock = 256 * kScanRows;                      // entries per wave and round
 constexpr long long kScanRound = kScanBlock * (kScanThreads / 64);     // entries per round
-__device__ __forceinline__ void scan_offsets(const DecodeArgs& a, long long* wsum)
+constexpr int kScanPartsMax = 64;                                      // workgroups per offsets launch
+
+// Sum of the table's first n entries (n a multiple of 4) by the whole
+// workgroup: 8-byte loads, a wave sum each, then the waves' sums.
+__device__ __forceinline__ long long table_prefix(const uint16_t* tab, long long n, long long* wsum)
+{
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  uint64_t acc = 0;
+  for (long long i = 4ll * t; i < n; i += 4ll * kScanThreads) {
+    const unsigned long long w = *(const unsigned long long*)(tab + i);   // 8-byte aligned: i is a multiple of 4
+    acc += (w & 0xffff) + ((w >> 16) & 0xffff) + ((w >> 32) & 0xffff) + (w >> 48);
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) wsum[wave] = (long long)acc;
+  __syncthreads();
+  long long tot = 0;
+#pragma unroll
+  for (int v = 0; v < kScanThreads / 64; v++) tot += wsum[v];
+  __syncthreads();   // wsum is reused by the rounds
+  return tot;
+}
+
+__device__ __forceinline__ void scan_offsets(const DecodeArgs& a, long long* wsum, int part, int parts)
 {
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   // this launch opens the call: it clears the status word (the later
   // kernels of the call only OR bits into it), so the call needs no
   // separate clearing launch
-  if (t == 0) flag_store(a.ws + kWsStatus, 0);
+  if (part == 0 && t == 0) flag_store(a.ws + kWsStatus, 0);
   const long long hdr = uni(*(const long long*)a.in);
   long long nchunks = 0;
   uint64_t why = 0;
   if (!header_ok(a, hdr, &nchunks, &why)) {
-    if (t == 0) raise_status(a.ws, why);
+    if (part == 0 && t == 0) raise_status(a.ws, why);
     return;
   }
-  if (t == 0) *a.decsize = hdr;
+  if (part == 0 && t == 0) *a.decsize = hdr;
   const uint16_t* const tab = (const uint16_t*)(a.in + 8);
   long long* const offs = (long long*)(a.ws + kWsFlags);
-  long long carry = 0;
+  const long long rounds = (nchunks + kScanRound - 1) / kScanRound;
+  const long long per = (rounds + parts - 1) / parts;   // rounds per part
+  const long long start = (long long)part * per * kScanRound;
+  const long long end = min(nchunks, start + per * kScanRound);
+  if (start >= nchunks && !(part == 0 && nchunks == 0)) return;   // uniform: no rounds here
+  long long carry = start > 0 ? table_prefix(tab, start, wsum) : 0;
   // Each wave takes kScanBlock consecutive entries per round, as kScanRows
   // rows of 256: lane l holds entries 4l .. 4l+3 of every row (one 8-byte
   // word), so the table loads and the offset stores are contiguous across
   // the wave.  A wave scan per row gives each lane its entries' offsets.
-  for (long long base = 0; base < nchunks; base += kScanRound) {
+  for (long long base = start; base < end; base += kScanRound) {
     const long long wb = base + (long long)wave * kScanBlock;
     unsigned long long w[kScanRows];
     uint32_t rs[kScanRows], incl[kScanRows];
@@ -782,7 +811,7 @@ __device__ __forceinline__ void scan_offsets(const DecodeArgs& a, long long* wsu
     carry += tile;
     __syncthreads();
   }
-  if (t == 0) offs[nchunks] = carry;
+  if (t == 0 && end == nchunks) offs[nchunks] = carry;   // the part holding the last entry (part 0 when there are none)
 }
 
 // One chunk of the decoder, by one workgroup of kThreads threads, once

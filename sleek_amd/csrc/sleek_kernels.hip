@@ -459,7 +459,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
 __global__ __launch_bounds__(kScanThreads) void offsets_kernel(DecodeArgs a)
 {
   __shared__ long long wsum[kScanThreads / 64];
-  scan_offsets(a, wsum);
+  scan_offsets(a, wsum, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // Decoder without a separate offset scan: workgroup b decodes chunks b,
@@ -533,7 +533,12 @@ template <typename W, bool LOSSY>
 static hipError_t launch_decode(const DecodeArgs& a, int grid, int variant, hipStream_t stream)
 {
   if (variant == kDecodeScan) {
-    hipLaunchKernelGGL(offsets_kernel, dim3(1), dim3(kScanThreads), 0, stream, a);
+    // parts of whole rounds, from the capacity's chunk count (the stream's own
+    // count is read by the kernel; parts beyond it have nothing to do), at
+    // most the decode grid
+    const long long rounds = ((a.cap + kChunk - 1) / kChunk + kScanRound - 1) / kScanRound;
+    const int parts = (int)std::max(1ll, std::min({(long long)kScanPartsMax, rounds, (long long)grid}));
+    hipLaunchKernelGGL(offsets_kernel, dim3(parts), dim3(kScanThreads), 0, stream, a);
     hipLaunchKernelGGL((decode_kernel<W, LOSSY>), dim3(grid), dim3(kThreads), 0, stream, a);
   } else {
     hipLaunchKernelGGL((decode_fused_kernel<W, LOSSY>), dim3(grid < kWinMax ? grid : kWinMax), dim3(kThreads), 0,

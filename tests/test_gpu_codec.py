@@ -270,9 +270,10 @@ def test_config_fields_scaled(gpu, config):
 
 @pytest.mark.parametrize("chunks", [2048, 2049, 32768, 32769, 70001])
 def test_many_chunks_round_trip(gpu, chunks):
-    """Chunk counts around the offset scan's blocks (2,048 table entries per wave)
-    and rounds (32,768 per pass), up to 1.1 GiB: lossless round trips are
-    bit-exact, and both decoders (offset scan and fused) give the same bytes."""
+    """Chunk counts around the offset scan's rounds (8,192 table entries each, one
+    workgroup per round up to 64), up to 1.1 GiB: lossless round trips are bit-exact,
+    and both decoders (offset scan and fused) give the same bytes; the offset scan
+    also at a decode grid of 3, whose 3 workgroups then take several rounds each."""
     import torch
     from sleek_amd import _lib, codec
 
@@ -285,17 +286,20 @@ def test_many_chunks_round_trip(gpu, chunks):
         for eb in (None, 1e-3):
             out, sz = codec.compress(x, eb)
             outs = []
-            for dec in (0, 1):
+            for dec, grid in ((0, 0), (1, 0), (0, 3)):
                 _lib.check(lib.sleek_gpu_set_decoder(dec), "decoder")
+                _lib.check(lib.sleek_gpu_set_tuning(0, 0, grid), "tuning")
                 d, dn = codec.decompress(out, x.dtype, eb, numel=n)
                 torch.cuda.synchronize()
                 assert int(dn.item()) == 4 * n
                 outs.append(d)
-            assert torch.equal(outs[0].view(torch.int32), outs[1].view(torch.int32)), eb
+            for d in outs[1:]:
+                assert torch.equal(outs[0].view(torch.int32), d.view(torch.int32)), eb
             if eb is None:
                 assert torch.equal(outs[0].view(torch.int32), x.view(torch.int32))
     finally:
         lib.sleek_gpu_set_decoder(0)
+        lib.sleek_gpu_set_tuning(0, 0, 0)
 
 
 @pytest.mark.parametrize("dt", [np.float32, np.float64])

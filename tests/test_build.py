@@ -89,6 +89,11 @@ def test_barriers_in_single_outer_loop(asm):
     for k in bpair:
         assert b[k] == [0, 0, 1, 1, 1, 0, 0], (k, b[k])
     fused += bpair
+    # the offset scan: the sum of the table entries before the workgroup's rounds
+    # (uniform: a launch constant), then per round the waves' sums and their reuse
+    offs = [k for k in b if "offsets_kernel" in k and "batch" not in k]
+    assert len(offs) == 1 and b[offs[0]] == [0, 0, 1, 1], b.get(offs[0] if offs else None)
+    fused += offs
     # single-wave workgroups (one-wave encoders, one-wave batched encoder) need no barrier instruction
     assert not [k for k in b if "wave" in k or ("batch_encode" in k and "pair" not in k)]
     assert all(d == 1 for k, v in b.items() if k not in fused for d in v), b

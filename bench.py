@@ -38,7 +38,7 @@ def parse():
     p.add_argument("--encode-variant", type=int, default=0,
                    help="0 default (6), 1 pipelined 256-thread, 2 the same at the compiler's register choice, "
                         "4 one wave per chunk pipelined over its chunks, 5 two waves per chunk, 6 two waves per "
-                        "chunk with chunk tickets")
+                        "chunk with chunk tickets, 7 one wave per chunk with chunk tickets (3 is retired)")
     p.add_argument("--encode-grid", type=int, default=0, help="encoder persistent grid (0 = automatic)")
     p.add_argument("--decode-grid", type=int, default=0, help="decoder grid (0 = one workgroup per chunk)")
     p.add_argument("--decoder", type=int, default=-1, help="decoder variant (-1 = library default)")
@@ -51,8 +51,9 @@ def parse():
     p.add_argument("--batch-steps", type=int, default=3)
     p.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     p.add_argument("--serve-requests", type=int, default=2000)
-    p.add_argument("--serve-rates", default="2000,10000,50000",
-                   help="Poisson arrival rates (requests/s) of the serving sweep; the first is the headline rate")
+    p.add_argument("--serve-rates", default="2000,10000,50000,200000",
+                   help="Poisson arrival rates (requests/s) of the serving sweep; the first is the headline rate "
+                        "(the last offers more than the GPU serves: its saturation point)")
     p.add_argument("--serve-max-mb", type=float, default=256.0)
     p.add_argument("--serve-budget-gb", type=float, default=96.0)
     return p.parse_args()
@@ -113,8 +114,9 @@ def bench_batch(args, dev):
     """The uneven batch: log-uniform 4 KiB .. 64 MiB arrays, 70 % float32, half
     lossless and half lossy at {1e-2, 1e-3, 1e-4} x range.  One graph-captured
     batched launch per direction against per-array plain launches (the async
-    C ABI in a loop on one stream, one reused workspace); both produce the
-    same bytes, which is checked.  Rates are aggregate input bytes / time."""
+    C ABI in a loop on one stream, each array with its own workspace, from a
+    C++ loop and from a Python loop); both forms produce the same bytes,
+    which is checked.  Rates are aggregate input bytes / time."""
     import ctypes
 
     import torch

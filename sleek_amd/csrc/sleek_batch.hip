@@ -239,8 +239,9 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   if (h == 0) publish_chunk(ec.a, ec.st, ec.c, ec.nchunks, sc.csize);
   int par = 1;
   long long n = TICKET ? uni(xnext) : t + gridDim.x;
+  const uint32_t one = ticket_one();
   while (TICKET ? n >= 0 : n < tiles) {
-    if (TICKET && h == 0) tq = ticket_request(tickets, lc.ticket_shift, tq);
+    if (TICKET && h == 0) tq = ticket_request(tickets, lc.ticket_shift, tq, one);
     const EncTile en = enc_tile(items, tile_map, n, ws, lc);
     uint2 nxt[kHalfSub];
     load_half<true>(en.a, en.c, h, nxt);

@@ -178,7 +178,11 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
       if (inc) {
         const int p = __ffsll((unsigned long long)inc) - 1;
         const uint64_t need = (2ull << p) - 1;      // p < 63 here
-        if ((rdy & need) == need) return wave_sum(lane <= p ? (f & kValMask) : 0);
+        if ((rdy & need) == need) {
+          const uint64_t r = wave_sum(lane <= p ? (f & kValMask) : 0);
+          wait_vm_drained();
+          return r;
+        }
       } else if (rdy == ~0ull) {
         partial = wave_sum(f & kValMask);
         have_partial = true;
@@ -216,7 +220,11 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
       if (inc) {
         const int p = __ffsll((unsigned long long)inc) - 1;
         const uint64_t need = (2ull << p) - 1;      // p < gwin <= 64; lanes >= gwin carry 0
-        if ((rdy & need) == need) return partial + gacc + wave_sum(lane <= p ? (fg & kValMask) : 0);
+        if ((rdy & need) == need) {
+          const uint64_t r = partial + gacc + wave_sum(lane <= p ? (fg & kValMask) : 0);
+          wait_vm_drained();
+          return r;
+        }
       } else if (rdy == ~0ull) {
         gacc += wave_sum(fg & kValMask);
         gend -= gwin;
@@ -467,16 +475,27 @@ __device__ __forceinline__ uint32_t ticket_limit(uint32_t nchunks, int x, int sh
 // result it would wait for at once.  Callers issue it before loads they wait
 // for early (memory operations complete in issue order as far as those waits
 // are concerned).
-__device__ __forceinline__ uint32_t ticket_request(uint32_t* tk, int shift, uint32_t prev)
+__device__ __forceinline__ uint32_t ticket_request(uint32_t* tk, int shift, uint32_t prev, uint32_t one = 1u)
 {
   uint32_t t = prev;
   if ((threadIdx.x & 63) == 0) {
     uintptr_t p = (uintptr_t)(tk + ticket_home(shift) * kTicketStride);
     asm volatile("" : "+v"(p));
-    t = __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t*)p, 1u, __ATOMIC_RELAXED,
+    t = __hip_atomic_fetch_add((__attribute__((address_space(1))) uint32_t*)p, one, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
   }
   return t;
+}
+
+// The increment of the encoders' in-loop ticket requests (1), in a register
+// of its own defined before the loop: otherwise the compiler materialises it
+// in the ticket's own register just before the atomic, and that write waits
+// for every memory operation in flight (the previous chunk's stores).
+__device__ __forceinline__ uint32_t ticket_one()
+{
+  uint32_t one = 1u;
+  asm volatile("" : "+v"(one));
+  return one;
 }
 
 // Once the workgroup's home set is spent: a ticket of the first other set
@@ -485,7 +504,7 @@ __device__ __forceinline__ uint32_t ticket_request(uint32_t* tk, int shift, uint
 // spent and each ticket below a limit goes to exactly one workgroup: every
 // chunk is taken.  A set whose ticket came back spent is not tried again, so
 // the loop runs at most S - 1 times whatever the reads return.
-__device__ __noinline__ long long ticket_fallback(uint32_t* tk, int shift, long long nchunks)
+__device__ __forceinline__ long long ticket_fallback(uint32_t* tk, int shift, long long nchunks)
 {
   typedef __attribute__((address_space(1))) uint32_t gu32;
   gu32* const g = (gu32*)tk;
@@ -675,14 +694,16 @@ __device__ __forceinline__ void pair_store(uint8_t* dst, const uint32_t* lds, in
   const int nfull = (len - head) >> 4;
   const int tail = len - head - (nfull << 4);
   const uint8_t* lb = (const uint8_t*)lds;
+  for (int j = h * 64 + lane; j < nfull; j += 128) {
+    const int p = head + (j << 4);
+    gstore((uint4*)(dst + p), lds_read16(lds, p));
+  }
+  // the edge bytes last: issued before the lines, their stores kept the
+  // line loop's registers waiting for the stores' completion
   if (h == 0 && lane < head) gstore(dst + lane, lb[lane]);
   if (h == 1 && lane < tail) {
     const int i = head + (nfull << 4) + lane;
     gstore(dst + i, lb[i]);
-  }
-  for (int j = h * 64 + lane; j < nfull; j += 128) {
-    const int p = head + (j << 4);
-    gstore((uint4*)(dst + p), lds_read16(lds, p));
   }
 }
 

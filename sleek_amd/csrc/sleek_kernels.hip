@@ -319,9 +319,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) void
   ChunkShape sc = shape_codes<W, LOSSY>(a, c, cur);
   publish_chunk(a, st, c, nchunks, sc.csize);
   long long n = TICKET ? ticket_chunk(a.tickets, a.ticket_shift, tq, nchunks) : c + gridDim.x;
+  const uint32_t one = ticket_one();
   while (true) {
     const bool more = TICKET ? n >= 0 : n < nchunks;
-    if (TICKET && more) tq = ticket_request(a.tickets, a.ticket_shift, tq);
+    if (TICKET && more) tq = ticket_request(a.tickets, a.ticket_shift, tq, one);
     uint2 nxt[kSub];
     if (more) load_raw(a, n, nxt);
     pack_codes<W, LOSSY>(a, c, cur, sc, stage);
@@ -409,10 +410,11 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   if (h == 0) publish_chunk(a, st, c, nchunks, sc.csize);
   long long n = TICKET ? uni(xnext) : c + gridDim.x;
   int par = 1;
+  const uint32_t one = ticket_one();
   while (true) {
     const bool more = TICKET ? n >= 0 : n < nchunks;
     uint2 nxt[kHalfSub];
-    if (TICKET && h == 0 && more) tq = ticket_request(a.tickets, a.ticket_shift, tq);
+    if (TICKET && h == 0 && more) tq = ticket_request(a.tickets, a.ticket_shift, tq, one);
     if (more) load_half(a, n, h, nxt);
     pack_half<W, LOSSY>(a, c, h, cur, sc, stage);
     __builtin_amdgcn_s_setprio(2);   // n's size is waited on by others: as in encode_wave2_kernel

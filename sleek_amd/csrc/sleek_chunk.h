@@ -180,7 +180,7 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         const uint64_t need = (2ull << p) - 1;      // p < 63 here
         if ((rdy & need) == need) {
           const uint64_t r = wave_sum(lane <= p ? (f & kValMask) : 0);
-          wait_vm_drained();
+          wait_vm_drained();   // free: the polls just used were issued last; nothing of the look-back stays pending
           return r;
         }
       } else if (rdy == ~0ull) {
@@ -222,7 +222,7 @@ __device__ uint64_t look_back(const EncodeArgs& a, const ScanState& s, long long
         const uint64_t need = (2ull << p) - 1;      // p < gwin <= 64; lanes >= gwin carry 0
         if ((rdy & need) == need) {
           const uint64_t r = partial + gacc + wave_sum(lane <= p ? (fg & kValMask) : 0);
-          wait_vm_drained();
+          wait_vm_drained();   // as above
           return r;
         }
       } else if (rdy == ~0ull) {
@@ -498,6 +498,11 @@ __device__ __forceinline__ uint32_t ticket_one()
   return one;
 }
 
+// Inlined: as a call it cost far more than its registers, because after a
+// call the compiler takes every register the callee may clobber as written
+// by a load still in flight, and carried around the encoders' loops that
+// put a wait for all memory in flight (the next chunk's prefetch included)
+// before later writes of those registers.
 // Once the workgroup's home set is spent: a ticket of the first other set
 // whose counter still has tickets (all others read at once, one per lane), or
 // -1 when every set is spent.  Counters only grow, so a set seen spent stays
@@ -532,8 +537,8 @@ __device__ __forceinline__ long long ticket_fallback(uint32_t* tk, int shift, lo
 // A workgroup's first chunk: a ticket of its home set (uniform), or -1 when
 // that set is already spent, and the workgroup then has nothing to do (the
 // chunks of every other set are taken by their own workgroups and their
-// fallbacks).  No fallback call here: a second call site of the out-of-line
-// fallback costs the encoders' loops register spills.
+// fallbacks).  No fallback here: a third inlined copy of it costs the
+// encoders' loops registers.
 __device__ __forceinline__ long long ticket_first(uint32_t* tk, int shift, long long nchunks)
 {
   const int home = ticket_home(shift);
@@ -698,8 +703,7 @@ __device__ __forceinline__ void pair_store(uint8_t* dst, const uint32_t* lds, in
     const int p = head + (j << 4);
     gstore((uint4*)(dst + p), lds_read16(lds, p));
   }
-  // the edge bytes last: issued before the lines, their stores kept the
-  // line loop's registers waiting for the stores' completion
+  // the edge bytes (disjoint from the lines: any order is correct)
   if (h == 0 && lane < head) gstore(dst + lane, lb[lane]);
   if (h == 1 && lane < tail) {
     const int i = head + (nfull << 4) + lane;

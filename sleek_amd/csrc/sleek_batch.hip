@@ -206,7 +206,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   __shared__ uint32_t xl[2][8];
   __shared__ unsigned long long xexcl;
   __shared__ long long xnext, xfirst;
-  const int h = threadIdx.x >> 6;
+  const int h = uni((int)(threadIdx.x >> 6));   // wave-uniform: scalar branches and bit lengths
   const int lane = threadIdx.x & 63;
   long long t = blockIdx.x;
   if (TICKET) {

@@ -592,7 +592,10 @@ __device__ __forceinline__ void shape_half(const EncodeArgs& a, long long c, int
   const W offs = (W)a.offs;
   const int osize = (int)min((long long)kChunk, a.insize - c * kChunk);
   if (osize < kChunk) {
-    const int rem = osize - h * kHalfSub * kSubBytes - (int)(threadIdx.x & 63) * 8;
+    // opaque, so the slices' limits are derived here rather than kept live
+    // (or spilled) across the encoders' loops as 16 per-lane constants
+    int rem = osize - h * kHalfSub * kSubBytes - (int)(threadIdx.x & 63) * 8;
+    asm volatile("" : "+v"(rem));
 #pragma unroll
     for (int s = 0; s < kHalfSub; s++) {
       const int r = rem - s * kSubBytes;
@@ -601,6 +604,17 @@ __device__ __forceinline__ void shape_half(const EncodeArgs& a, long long c, int
     }
   }
   uint32_t ov[kHalfSub];
+  if constexpr (LOSSY && sizeof(W) == 4) {
+    const LossyF32 p = lossy_f32_params(a.thr_e, (uint32_t)a.offs);
+#pragma unroll
+    for (int s = 0; s < kHalfSub; s++) {
+      const uint32_t c0 = lossy_code_f32(slot[s].x, p), c1 = lossy_code_f32(slot[s].y, p);
+      slot[s] = make_uint2(c0, c1);
+      ov[s] = c0 | c1;
+    }
+    wave_lengths<kHalfSub, false>(ov, Lh);
+    return;
+  }
 #pragma unroll
   for (int s = 0; s < kHalfSub; s++) {
     W v[PL];
@@ -660,11 +674,11 @@ __device__ __forceinline__ void pack_half(const EncodeArgs& a, long long c, int 
     int p0 = h == 0 ? 8 * kSub : mid;
 #pragma unroll
     for (int s = 0; s < kHalfSub; s++) {
-      const int ln = sub_length(lw, s);
+      const int ln = sub_length(lw, s);   // uniform (h is)
       if (ln) {
         W v[PL];
         lane_values(code[s], v);
-        or_field(stage, p0 + lane * PL * ln, lane_field(v, ln));
+        or_codes(stage, p0, lane, ln, v);
       }
       p0 += ln * Word<W>::PerSub;
     }

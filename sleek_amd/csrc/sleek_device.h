@@ -7,6 +7,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace sleek {
 
 // Loads and stores through global-address-space pointers.  Pointers that
@@ -113,8 +115,10 @@ __host__ __device__ __forceinline__ W lossless_code(W x)
   constexpr int TB = Word<W>::TB;
   constexpr W kOne = (W)1 << (Word<W>::MANB + 1);
   const W u = ((x << 1) | (x >> (TB - 1))) - kOne;
-  const W t = u >> (TB - 1);
-  return ((u << 1) + (t << (Word<W>::MANB + 2))) ^ (t - 1);
+  // ts = -t (0 or all ones): t << (M + 2) = ts & 2^(M+2), t - 1 = ~ts
+  typedef typename std::conditional<sizeof(W) == 4, int32_t, int64_t>::type S;
+  const W ts = (W)((S)u >> (TB - 1));
+  return ((u << 1) + (ts & ((W)1 << (Word<W>::MANB + 2)))) ^ ~ts;
 }
 
 // Inverse of lossless_code (reference sleek_common_cpu.h:317-335): bit 0
@@ -160,6 +164,64 @@ __host__ __device__ __forceinline__ W lossy_code(W val, int eb_e, int thr_e, W o
   const W q = (e >= thr_e) ? (W)(mag - offs) : binned;
   const W sign = val >> (TB - 1);
   return (q << 1) - (q < sign ? q : sign);
+}
+
+// Float32 error-bounded code: the same function as lossy_code<uint32_t> in
+// 10 instructions instead of 16, for every bound (thr_e in [24, 254]).
+//  * Bin: for e < thr_e, (mant >> sh1) = floor(|x| * 2^(151 - thr_e)) for
+//    normal x, the product being exact (a power-of-two scaling of a value
+//    that lands below 2^24; one that underflows is below 1 and truncates to 0
+//    either way, flushed or not).  Zero and denormal inputs (e = 0): the
+//    reference's (frac | 2^23) >> sh1 with sh1 = thr_e - 1 is 1 when
+//    thr_e = 24 (a denormal bound) and 0 above, while the truncated product
+//    is 0 or 1; a floor tmin (1 for thr_e = 24, else 0) makes them equal, and
+//    no normal input is affected (its truncation is >= 2 when thr_e = 24).
+//    So 2 * binned = (t + 1) & ~1 with t = max(truncation, tmin); the
+//    truncation is v_cvt_u32_f32, which saturates and maps NaN to 0 (such
+//    lanes take the exact branch).
+//  * Exact branch (|x| >= 2^(thr_e - 127), Inf and NaN: !(|x| < thr)):
+//    2 * (mag - offs) = (val << 1) - 2 offs modulo 2^32 (the sign bit shifts out).
+//  * Sign fold: 2q - min(q, sign) = 2q -sat sign (2q is 0 exactly when q is).
+struct LossyF32 {
+  float scale;        // 2^(151 - thr_e)
+  float thr;          // 2^(thr_e - 127): the exact branch's threshold
+  uint32_t negoffs2;  // -(2 offs) mod 2^32
+  uint32_t tmin;      // 1 for a denormal bound (thr_e = 24), else 0
+};
+
+__host__ __device__ __forceinline__ LossyF32 lossy_f32_params(int thr_e, uint32_t offs)
+{
+  LossyF32 p;
+  const uint32_t sb = (uint32_t)(278 - thr_e) << 23, tb = (uint32_t)thr_e << 23;
+  __builtin_memcpy(&p.scale, &sb, 4);
+  __builtin_memcpy(&p.thr, &tb, 4);
+  p.negoffs2 = 0u - (offs << 1);
+  p.tmin = thr_e == 24 ? 1u : 0u;
+  return p;
+}
+
+__host__ __device__ __forceinline__ uint32_t lossy_code_f32(uint32_t val, const LossyF32& p)
+{
+  float x;
+  __builtin_memcpy(&x, &val, 4);
+  const float ax = __builtin_fabsf(x);
+  const float y = ax * p.scale;
+  uint32_t t;
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("v_cvt_u32_f32 %0, %1" : "=v"(t) : "v"(y));   // truncating, saturating: no undefined conversion
+#else
+  t = (y != y) ? 0u : (y >= 4294967296.0f ? 0xffffffffu : (uint32_t)y);
+#endif
+  t = t > p.tmin ? t : p.tmin;
+  const uint32_t b2 = (t + 1u) & ~1u;
+  uint32_t no2 = p.negoffs2;
+#if defined(__HIP_DEVICE_COMPILE__)
+  asm("" : "+s"(no2));   // opaque: kept as an addend (one v_lshl_add_u32), not folded into a subtraction
+#endif
+  const uint32_t e2 = (val << 1) + no2;
+  const uint32_t q2 = !(ax < p.thr) ? e2 : b2;
+  const uint32_t s = val >> 31;
+  return q2 > s ? q2 - s : 0u;   // saturating subtract
 }
 
 // Inverse of lossy_code (reference sleek_common_cpu.h:559-591), branch-free.
@@ -438,6 +500,36 @@ __device__ __forceinline__ void or_field(uint32_t* st, int p, uint64_t f)
   atomicOr(&st[d], (uint32_t)lo);
   atomicOr(&st[d + 1], (uint32_t)(lo >> 32));
   if (w2) atomicOr(&st[d + 2], w2);
+}
+
+// OR a lane's codes of one subchunk into a zeroed LDS bit buffer whose
+// fields start at bit p0 (a multiple of 32).  ln (1..TB) is wave-uniform, so
+// the case is a scalar branch: the lane's field (PL codes of ln bits, PL * ln
+// bits) goes to bit p0 + PL * ln * lane.
+//  * float32, ln <= 16: the 2 ln-bit field fits a dword; when 2 ln divides
+//    32 no field straddles a dword boundary (one OR), else two ORs.
+//  * otherwise: a field of up to 64 bits over up to three dwords (or_field).
+__device__ __forceinline__ void or_codes(uint32_t* st, int p0, int lane, int ln, const uint32_t v[2])
+{
+  const uint32_t x = (uint32_t)lane * (uint32_t)(2 * ln);
+  if (ln <= 16) {
+    const uint32_t sh = x & 31u;
+    uint32_t* const d = st + (p0 >> 5) + (x >> 5);
+    const uint32_t f = v[0] | (v[1] << ln);
+    if ((ln & (ln - 1)) == 0) {
+      atomicOr(d, f << sh);
+    } else {
+      const uint64_t g = (uint64_t)f << sh;
+      atomicOr(d, (uint32_t)g);
+      atomicOr(d + 1, (uint32_t)(g >> 32));
+    }
+  } else {
+    or_field(st, p0 + (int)x, (uint64_t)v[0] | ((uint64_t)v[1] << ln));
+  }
+}
+__device__ __forceinline__ void or_codes(uint32_t* st, int p0, int lane, int ln, const uint64_t v[1])
+{
+  or_field(st, p0 + lane * ln, v[0]);
 }
 
 // The 64 bits of an LDS bit buffer starting at bit p.

@@ -1,8 +1,8 @@
 // Host-side check of the codec's value transforms (sleek_amd/csrc/sleek_device.h,
 // the very functions the kernels inline) against the oracle: lossless code
-// and inverse, and error-bounded code and inverse at bounds of every class
-// (denormal, tiny, ordinary, the largest allowed), over all 2^32 float32 bit
-// patterns and over random float64 patterns covering every exponent.
+// and inverse, and error-bounded code (both forms, integer and float) and
+// inverse at bounds of every class (denormal, tiny, ordinary, the largest
+// allowed), over all 2^32 float32 bit patterns and over random float64 patterns covering every exponent.
 // Exit status 0 = no mismatch.
 #include <stdint.h>
 #include <stdio.h>
@@ -20,7 +20,7 @@ struct Bound64 { int e, t; uint64_t o; };
 int main()
 {
   std::vector<Bound32> b32;
-  for (float eb : {1e-40f, 1e-30f, 1e-3f, 1.0f, 9.0e30f}) {
+  for (float eb : {1e-40f, 1.5e-38f, 1e-30f, 1e-3f, 1.0f, 9.0e30f}) {
     Bound32 b;
     if (sleek_oracle_error_bound_f32(eb, &b.e, &b.t, &b.o) != 0) return 2;
     b32.push_back(b);
@@ -45,6 +45,7 @@ int main()
         for (const Bound32& q : b32) {
           const uint32_t qc = sleek::lossy_code<uint32_t>(x, q.e, q.t, q.o);
           b += (qc != sleek_oracle_quantize_u32(x, q.e, q.t, q.o));
+          b += (sleek::lossy_code_f32(x, sleek::lossy_f32_params(q.t, q.o)) != qc);   // the float form
           b += (sleek::lossy_value<uint32_t>(qc, q.t, q.o) != sleek_oracle_dequantize_u32(qc, q.t, q.o));
         }
       }

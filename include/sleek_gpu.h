@@ -128,6 +128,15 @@ int sleek_gpu_set_help_after(int microseconds);
  * current device since the library was loaded; synchronises the device.
  * Returns the count, or a negative status. */
 long long sleek_gpu_help_events(void);
+/* Kernel timing (no reference counterpart; for benchmarks): the next
+ * compress or decompress call (single-array, sync or async) that this host
+ * thread makes records the caller's HIP events (hipEvent_t, either may be
+ * null) on the call's stream: `begin` right before its coding kernel (the
+ * encoder, or the decoder after its offset scan) and `end` right after it,
+ * so hipEventElapsedTime(begin, end) is that kernel's own duration, without
+ * the workspace clear or the offset scan.  The setting is used once and
+ * then cleared.  Returns SLEEK_OK. */
+int sleek_gpu_set_kernel_events(void* begin, void* end);
 /* Look-back geometry of the encoders (no reference counterpart; the
  * reference watches a fixed window of 32 chunks, sleek_common_gpu.h:220-253):
  * group_chunks = chunks per look-back group, 8, 16, 32 or 64 (0 = default,

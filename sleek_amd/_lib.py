@@ -51,6 +51,7 @@ SIGNATURES = [
     ("sleek_gpu_set_decoder", c_i, [c_i]),
     ("sleek_gpu_set_help_after", c_i, [c_i]),
     ("sleek_gpu_help_events", c_ll, []),
+    ("sleek_gpu_set_kernel_events", c_i, [c_p, c_p]),
     ("sleek_gpu_workspace_status", c_i, [c_p, c_p]),
     ("sleek_gpu_compress_noalloc", c_i, [c_i, c_i, c_i, c_i, c_ull, c_p, c_ll, c_p, c_p, c_i, c_p]),
     ("sleek_gpu_batch_plan", c_i, [c_p, c_i, c_i, ctypes.POINTER(c_ll), ctypes.POINTER(c_ll)]),

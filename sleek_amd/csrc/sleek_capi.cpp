@@ -83,6 +83,20 @@ int scratch_for(long long need, void** out)
   return SLEEK_OK;
 }
 
+// sleek_gpu_set_kernel_events: events the calling thread's next codec call
+// records around its coding kernel (used once, then cleared).
+struct KernelEvents {
+  hipEvent_t begin = nullptr, end = nullptr;
+};
+thread_local KernelEvents t_kernel_events;
+
+KernelEvents take_kernel_events()
+{
+  const KernelEvents k = t_kernel_events;
+  t_kernel_events = KernelEvents();
+  return k;
+}
+
 int grid_for(int blocks, int elem_bytes, bool lossy, bool decode, long long nchunks)
 {
   long long g = blocks;
@@ -131,8 +145,11 @@ int compress_impl(int elem_bytes, bool lossy, int eb_e, int thr_e, unsigned long
   a.help = sleek::help_state();
   const int grid = grid_for(blocks, elem_bytes, lossy, false, chunks_of(insize));
   a.ticket_shift = sleek::ticket_shift_for(grid);
+  const KernelEvents ev = take_kernel_events();
+  if (ev.begin && (e = hipEventRecord(ev.begin, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   e = sleek::encode_launch(elem_bytes, lossy, a, grid, g_encode_variant, stream);
   if (e != hipSuccess) return hip_fail(e, "encode launch");
+  if (ev.end && (e = hipEventRecord(ev.end, stream)) != hipSuccess) return hip_fail(e, "hipEventRecord");
   return SLEEK_OK;
 }
 
@@ -168,7 +185,8 @@ int decompress_impl(int elem_bytes, bool lossy, int thr_e, unsigned long long of
   a.thr_e = thr_e;
   a.offs = offs;
   const int grid = grid_for(blocks, elem_bytes, lossy, true, chunks_of(cap));
-  e = sleek::decode_launch(elem_bytes, lossy, a, grid, variant, stream);
+  const KernelEvents ev = take_kernel_events();
+  e = sleek::decode_launch(elem_bytes, lossy, a, grid, variant, stream, ev.begin, ev.end);
   if (e != hipSuccess) return hip_fail(e, "decode launch");
   return SLEEK_OK;
 }
@@ -376,6 +394,13 @@ int sleek_gpu_set_help_after(int microseconds)
   if (microseconds < 0 || microseconds > 10000000) return SLEEK_EINVAL;
   const hipError_t e = sleek::set_help_naps((unsigned long long)microseconds * sleek::kHelpNapsPerUs);
   return e == hipSuccess ? SLEEK_OK : hip_fail(e, "help threshold");
+}
+
+int sleek_gpu_set_kernel_events(void* begin, void* end)
+{
+  t_kernel_events.begin = (hipEvent_t)begin;
+  t_kernel_events.end = (hipEvent_t)end;
+  return SLEEK_OK;
 }
 
 long long sleek_gpu_help_events(void)

@@ -135,7 +135,10 @@ hipError_t zero_launch(uint64_t* p, long long words, hipStream_t stream);
 constexpr int kDecodeScan = 0;
 constexpr int kDecodeFused = 1;
 constexpr int kDecodeVariants = 2;
-hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, int variant, hipStream_t stream);
+// ev_begin / ev_end (may be null): recorded on the stream right before and
+// right after the decoder kernel (after the offset scan).
+hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, int variant, hipStream_t stream,
+                         hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 int resident_grid(int elem_bytes, bool lossy, bool decode, int variant);
 
 // One array of a batched call; the layout of sleek_gpu_batch_item.

@@ -532,8 +532,10 @@ static hipError_t launch_encode(const EncodeArgs& a, int grid, int variant, hipS
 }
 
 template <typename W, bool LOSSY>
-static hipError_t launch_decode(const DecodeArgs& a, int grid, int variant, hipStream_t stream)
+static hipError_t launch_decode(const DecodeArgs& a, int grid, int variant, hipStream_t stream, hipEvent_t ev_begin,
+                                hipEvent_t ev_end)
 {
+  hipError_t e = hipSuccess;
   if (variant == kDecodeScan) {
     // parts of whole rounds, from the capacity's chunk count (the stream's own
     // count is read by the kernel; parts beyond it have nothing to do), at
@@ -541,12 +543,16 @@ static hipError_t launch_decode(const DecodeArgs& a, int grid, int variant, hipS
     const long long rounds = ((a.cap + kChunk - 1) / kChunk + kScanRound - 1) / kScanRound;
     const int parts = (int)std::max(1ll, std::min({(long long)kScanPartsMax, rounds, (long long)grid}));
     hipLaunchKernelGGL(offsets_kernel, dim3(parts), dim3(kScanThreads), 0, stream, a);
+    if (ev_begin && (e = hipEventRecord(ev_begin, stream)) != hipSuccess) return e;
     hipLaunchKernelGGL((decode_kernel<W, LOSSY>), dim3(grid), dim3(kThreads), 0, stream, a);
   } else {
+    if (ev_begin && (e = hipEventRecord(ev_begin, stream)) != hipSuccess) return e;
     hipLaunchKernelGGL((decode_fused_kernel<W, LOSSY>), dim3(grid < kWinMax ? grid : kWinMax), dim3(kThreads), 0,
                        stream, a);
   }
-  return hipGetLastError();
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if (ev_end) e = hipEventRecord(ev_end, stream);
+  return e;
 }
 
 hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int grid, int variant, hipStream_t stream)
@@ -555,11 +561,14 @@ hipError_t encode_launch(int elem_bytes, bool lossy, const EncodeArgs& a, int gr
   return lossy ? launch_encode<uint64_t, true>(a, grid, variant, stream) : launch_encode<uint64_t, false>(a, grid, variant, stream);
 }
 
-hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, int variant, hipStream_t stream)
+hipError_t decode_launch(int elem_bytes, bool lossy, const DecodeArgs& a, int grid, int variant, hipStream_t stream,
+                         hipEvent_t ev_begin, hipEvent_t ev_end)
 {
   if (elem_bytes == 4)
-    return lossy ? launch_decode<uint32_t, true>(a, grid, variant, stream) : launch_decode<uint32_t, false>(a, grid, variant, stream);
-  return lossy ? launch_decode<uint64_t, true>(a, grid, variant, stream) : launch_decode<uint64_t, false>(a, grid, variant, stream);
+    return lossy ? launch_decode<uint32_t, true>(a, grid, variant, stream, ev_begin, ev_end)
+                 : launch_decode<uint32_t, false>(a, grid, variant, stream, ev_begin, ev_end);
+  return lossy ? launch_decode<uint64_t, true>(a, grid, variant, stream, ev_begin, ev_end)
+               : launch_decode<uint64_t, false>(a, grid, variant, stream, ev_begin, ev_end);
 }
 
 template <typename W, bool LOSSY>

@@ -224,7 +224,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   EncTile ec = enc_tile(items, tile_map, t, ws, lc);
   uint2 cur[kHalfSub];
   uint32_t lh[4];
-  load_half<true>(ec.a, ec.c, h, cur);
+  load_half(ec.a, ec.c, h, cur);
   enc_shape_half(ec, h, cur, lh);
   if (lane == 0) {
 #pragma unroll
@@ -244,7 +244,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     if (TICKET && h == 0) tq = ticket_request(tickets, lc.ticket_shift, tq, one);
     const EncTile en = enc_tile(items, tile_map, n, ws, lc);
     uint2 nxt[kHalfSub];
-    load_half<true>(en.a, en.c, h, nxt);
+    load_half(en.a, en.c, h, nxt);
     enc_pack_half(ec, h, cur, sc, stage);
     __builtin_amdgcn_s_setprio(2);   // others' look-backs wait on this size
     enc_shape_half(en, h, nxt, lh);

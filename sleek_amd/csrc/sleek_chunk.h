@@ -24,10 +24,13 @@ __device__ __forceinline__ uint4 load_slice(const uint8_t* src, int off, int osi
   return q;
 }
 
+// NT: the non-temporal hint on whole slices (see ntload)
+template <bool NT = false>
 __device__ __forceinline__ void store_slice(uint8_t* dst, int off, int osize, uint4 q)
 {
   if (off + 16 <= osize) {
-    *(uint4*)(dst + off) = q;
+    if (NT) ntstore((uint4*)(dst + off), q);
+    else *(uint4*)(dst + off) = q;
     return;
   }
   const uint32_t d[4] = {q.x, q.y, q.z, q.w};
@@ -71,10 +74,15 @@ __device__ __forceinline__ uint2 load_lane_rem(const uint2* lp, const uint2* saf
   return q;
 }
 
+template <bool NT = false>
 __device__ __forceinline__ void store_lane(uint8_t* dst, int off, int osize, uint2 q)
 {
-  if (off + 8 <= osize) *(uint2*)(dst + off) = q;
-  else if (off < osize) *(uint32_t*)(dst + off) = q.x;
+  if (off + 8 <= osize) {
+    if (NT) ntstore((uint2*)(dst + off), q);
+    else *(uint2*)(dst + off) = q;
+  } else if (off < osize) {
+    *(uint32_t*)(dst + off) = q.x;
+  }
 }
 
 // Compressed size of chunk j, computed by one whole wave from the input (the
@@ -276,7 +284,7 @@ __device__ __forceinline__ void load_raw(const EncodeArgs& a, long long c, uint2
   if (osize == kChunk) {
     // full chunk: one pointer, constant offsets, no masking
 #pragma unroll
-    for (int s = 0; s < kSub; s++) raw[s] = gload(p + s * (kSubBytes / 8));
+    for (int s = 0; s < kSub; s++) raw[s] = ntload(p + s * (kSubBytes / 8));   // streamed: see ntload
   } else {
     // an opaque copy of the lane's valid byte count chooses the addresses:
     // the compiler cannot tie it to the masking, so it cannot turn the
@@ -285,7 +293,7 @@ __device__ __forceinline__ void load_raw(const EncodeArgs& a, long long c, uint2
     asm volatile("" : "+v"(rem));
     const uint2* const safe = (const uint2*)(a.in + base);
 #pragma unroll
-    for (int s = 0; s < kSub; s++) raw[s] = gload(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
+    for (int s = 0; s < kSub; s++) raw[s] = ntload(rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe);
     wait_vm_drained();
   }
 }
@@ -556,9 +564,9 @@ __device__ __forceinline__ long long ticket_chunk(uint32_t* tk, int shift, uint3
   return ticket_fallback(tk, shift, nchunks);
 }
 
-// GLOBAL: the array pointer was read from memory (a batch item): load
-// through global-address-space pointers (gload).
-template <bool GLOBAL = false>
+// The wave's half of chunk c's slices, streamed (ntload), through
+// global-address-space pointers also where the array pointer was read from
+// memory (a batch item), whose accesses would otherwise be flat.
 __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int h, uint2 raw[kHalfSub])
 {
   // opaque: the lane's offset is recomputed here rather than kept live (or
@@ -569,7 +577,7 @@ __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int 
   const uint2* const p = (const uint2*)(a.in + base + h * kHalfSub * kSubBytes) + lane;
   if (osize == kChunk) {
 #pragma unroll
-    for (int s = 0; s < kHalfSub; s++) raw[s] = GLOBAL ? gload(p + s * (kSubBytes / 8)) : p[s * (kSubBytes / 8)];
+    for (int s = 0; s < kHalfSub; s++) raw[s] = ntload(p + s * (kSubBytes / 8));   // streamed: see ntload
   } else {
     int rem = osize - h * kHalfSub * kSubBytes - lane * 8;   // opaque: see load_raw
     asm volatile("" : "+v"(rem));
@@ -577,7 +585,7 @@ __device__ __forceinline__ void load_half(const EncodeArgs& a, long long c, int 
 #pragma unroll
     for (int s = 0; s < kHalfSub; s++) {
       const uint2* const q = rem > s * kSubBytes ? p + s * (kSubBytes / 8) : safe;
-      raw[s] = GLOBAL ? gload(q) : *q;
+      raw[s] = ntload(q);
     }
     wait_vm_drained();
   }
@@ -882,7 +890,9 @@ __device__ __forceinline__ void decode_chunk_at(const DecodeArgs& a, long long h
     const int mis = (int)((uintptr_t)src & 15);
     const uint4* const cover = (const uint4*)(src - mis);
     const int nblk = (mis + csize + 15) >> 4;
-    for (int j = t; j < nblk; j += kThreads) ((uint4*)stage)[j] = cover[j];
+    // streamed payload reads for lossless streams, streamed output stores for
+    // lossy ones (each measured best for its mode; see ntload)
+    for (int j = t; j < nblk; j += kThreads) ((uint4*)stage)[j] = LOSSY ? cover[j] : ntload(cover + j);
     __syncthreads();
 
     uint8_t* const dst = a.out + base;
@@ -890,7 +900,7 @@ __device__ __forceinline__ void decode_chunk_at(const DecodeArgs& a, long long h
 #pragma unroll
       for (int r = 0; r < kRounds; r++) {
         const int off = r * kThreads * 16 + t * 16;
-        if (off < osize) store_slice(dst, off, osize, lds_read16(stage, mis + off));
+        if (off < osize) store_slice<LOSSY>(dst, off, osize, lds_read16(stage, mis + off));
       }
     } else {
       // 32 bit-length bytes, realigned from the cover
@@ -927,7 +937,7 @@ __device__ __forceinline__ void decode_chunk_at(const DecodeArgs& a, long long h
             const uint64_t m = ln >= 64 ? ~0ull : ((1ull << ln) - 1ull);
             v[0] = from_code<W, LOSSY>((W)(f & m), a.thr_e, offs);
           }
-          store_lane(dst, woff + r * kSubBytes, osize, values_lane(v));
+          store_lane<LOSSY>(dst, woff + r * kSubBytes, osize, values_lane(v));
         }
       }
     }

@@ -37,6 +37,30 @@ __device__ __forceinline__ T gload(const T* p)
   __builtin_memcpy(&t, &r, sizeof(T));
   return t;
 }
+// The same with the non-temporal hint (the `nt` bit: stream through the
+// caches instead of keeping the line).  The encoders read their input this
+// way: read once, 2x the caches' size, it would otherwise evict the lines
+// the look-back polls and the counters live on (measured 7-14 % faster
+// encoders on 512^3 float32).  Lossy decoders store their output this way
+// (about 12 % faster); the lossless decoder reads its payload this way.
+template <typename T>
+__device__ __forceinline__ T ntload(const T* p)
+{
+  typedef typename GRaw<sizeof(T)>::T R;
+  const R r = __builtin_nontemporal_load((const __attribute__((address_space(1))) R*)p);
+  T t;
+  __builtin_memcpy(&t, &r, sizeof(T));
+  return t;
+}
+template <typename T>
+__device__ __forceinline__ void ntstore(T* p, T v)
+{
+  typedef typename GRaw<sizeof(T)>::T R;
+  R r;
+  __builtin_memcpy(&r, &v, sizeof(T));
+  __builtin_nontemporal_store(r, (__attribute__((address_space(1))) R*)p);
+}
+
 template <typename T>
 __device__ __forceinline__ void gstore(T* p, T v)
 {

@@ -369,3 +369,43 @@ def test_cpp_dropin_headers(gpu, tmp_path, dt, n, eb):
     assert open(so, "rb").read() == ref.tobytes()
     od, rc = po.decompress(ref, npdt, ebv)
     assert rc == 0 and open(do, "rb").read() == od.tobytes()
+
+
+def test_kernel_events_bracket_the_coding_kernels(gpu):
+    """sleek_gpu_set_kernel_events: the next call records the caller's events right
+    around its coding kernel (encoder; decoder after its offset scan), once; the bytes
+    are unchanged, the kernel's time is positive and within the whole call's, and a
+    later call records nothing (the setting is used once)."""
+    say("kernel events around the encoder and the decoder")
+    import torch
+    from sleek_amd import _lib, codec
+    from sleek_amd.datagen import exact_field
+
+    lib = _lib.load()
+    a = exact_field(4096 * 300 + 5, 23, np.float32)
+    x = torch.from_numpy(a).cuda()
+    ref = po.compress(a, 1e-3)
+    st = torch.cuda.current_stream()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(8)]
+    for e in ev:
+        e.record(st)   # creates the HIP events
+    torch.cuda.synchronize()
+    ev[0].record(st)
+    _lib.check(lib.sleek_gpu_set_kernel_events(ev[2].cuda_event, ev[3].cuda_event), "events")
+    out, n = codec.compress(x, 1e-3)
+    ev[1].record(st)
+    _lib.check(lib.sleek_gpu_set_kernel_events(ev[6].cuda_event, ev[7].cuda_event), "events")
+    d, _ = codec.decompress(out, x.dtype, 1e-3, numel=a.size)
+    ev[4].record(st)
+    torch.cuda.synchronize()
+    assert out[: int(n.item())].cpu().numpy().tobytes() == ref.tobytes()
+    od, rc = po.decompress(ref, np.float32, 1e-3)
+    assert rc == 0 and d.cpu().numpy().tobytes() == od.tobytes()
+    k_enc, call_enc = ev[2].elapsed_time(ev[3]), ev[0].elapsed_time(ev[1])
+    k_dec, call_dec = ev[6].elapsed_time(ev[7]), ev[1].elapsed_time(ev[4])
+    assert 0 < k_enc <= call_enc and 0 < k_dec <= call_dec, (k_enc, call_enc, k_dec, call_dec)
+    # used once: a second call leaves the events where the first one recorded them
+    ev[5].record(st)
+    codec.compress(x, 1e-3)
+    torch.cuda.synchronize()
+    assert ev[7].elapsed_time(ev[5]) > 0   # the decoder's events were not recorded again after ev[5]

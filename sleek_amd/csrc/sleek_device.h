@@ -191,36 +191,36 @@ __host__ __device__ __forceinline__ W lossy_code(W val, int eb_e, int thr_e, W o
 }
 
 // Float32 error-bounded code: the same function as lossy_code<uint32_t> in
-// 10 instructions instead of 16, for every bound (thr_e in [24, 254]).
-//  * Bin: for e < thr_e, (mant >> sh1) = floor(|x| * 2^(151 - thr_e)) for
-//    normal x, the product being exact (a power-of-two scaling of a value
-//    that lands below 2^24; one that underflows is below 1 and truncates to 0
-//    either way, flushed or not).  Zero and denormal inputs (e = 0): the
-//    reference's (frac | 2^23) >> sh1 with sh1 = thr_e - 1 is 1 when
-//    thr_e = 24 (a denormal bound) and 0 above, while the truncated product
-//    is 0 or 1; a floor tmin (1 for thr_e = 24, else 0) makes them equal, and
-//    no normal input is affected (its truncation is >= 2 when thr_e = 24).
-//    So 2 * binned = (t + 1) & ~1 with t = max(truncation, tmin); the
-//    truncation is v_cvt_u32_f32, which saturates and maps NaN to 0 (such
-//    lanes take the exact branch).
+// 9 instructions instead of 16, for every bound (thr_e in [24, 254]).
+//  * Bin: for e < thr_e, (mant >> sh1) = floor(y) with y = |x| * 2^(151 - thr_e)
+//    for normal x, the product being exact (a power-of-two scaling of a value
+//    that lands below 2^24; one that underflows is below 1 either way,
+//    flushed or not).  The bin is ((mant >> sh1) + 1) >> 1 = floor((y + 1) / 2)
+//    = floor(z + 1/2) with z = y / 2 = |x| * 2^(150 - thr_e), again exact:
+//    that is one v_cvt_rpi_i32_f32 (floor(z + 0.5), taken exactly).  Zero and
+//    denormal inputs (e = 0): the reference's bin is 1 when thr_e = 24 (a
+//    denormal bound) and 0 above, while floor(z + 1/2) is 0 or 1; a floor
+//    rmin (1 for thr_e = 24, else 0) makes them equal, and no normal input is
+//    affected (its bin is >= 1 when thr_e = 24).
 //  * Exact branch (|x| >= 2^(thr_e - 127), Inf and NaN: !(|x| < thr)):
-//    2 * (mag - offs) = (val << 1) - 2 offs modulo 2^32 (the sign bit shifts out).
+//    q = mag - offs = val - offs modulo 2^31; the doubling below shifts the
+//    sign bit (and the borrow into it) out.
 //  * Sign fold: 2q - min(q, sign) = 2q -sat sign (2q is 0 exactly when q is).
 struct LossyF32 {
-  float scale;        // 2^(151 - thr_e)
+  float hscale;       // 2^(150 - thr_e)
   float thr;          // 2^(thr_e - 127): the exact branch's threshold
-  uint32_t negoffs2;  // -(2 offs) mod 2^32
-  uint32_t tmin;      // 1 for a denormal bound (thr_e = 24), else 0
+  uint32_t negoffs;   // -offs mod 2^32
+  int rmin;           // 1 for a denormal bound (thr_e = 24), else 0
 };
 
 __host__ __device__ __forceinline__ LossyF32 lossy_f32_params(int thr_e, uint32_t offs)
 {
   LossyF32 p;
-  const uint32_t sb = (uint32_t)(278 - thr_e) << 23, tb = (uint32_t)thr_e << 23;
-  __builtin_memcpy(&p.scale, &sb, 4);
+  const uint32_t sb = (uint32_t)(277 - thr_e) << 23, tb = (uint32_t)thr_e << 23;
+  __builtin_memcpy(&p.hscale, &sb, 4);
   __builtin_memcpy(&p.thr, &tb, 4);
-  p.negoffs2 = 0u - (offs << 1);
-  p.tmin = thr_e == 24 ? 1u : 0u;
+  p.negoffs = 0u - offs;
+  p.rmin = thr_e == 24 ? 1 : 0;
   return p;
 }
 
@@ -229,22 +229,17 @@ __host__ __device__ __forceinline__ uint32_t lossy_code_f32(uint32_t val, const 
   float x;
   __builtin_memcpy(&x, &val, 4);
   const float ax = __builtin_fabsf(x);
-  const float y = ax * p.scale;
-  uint32_t t;
+  const float z = ax * p.hscale;
+  int r;
 #if defined(__HIP_DEVICE_COMPILE__)
-  asm("v_cvt_u32_f32 %0, %1" : "=v"(t) : "v"(y));   // truncating, saturating: no undefined conversion
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(z));   // floor(z + 0.5), saturating, NaN to 0
 #else
-  t = (y != y) ? 0u : (y >= 4294967296.0f ? 0xffffffffu : (uint32_t)y);
+  const double zr = __builtin_floor((double)z + 0.5);   // exact where it is used (z < 2^23)
+  r = (z != z) ? 0 : (zr >= 2147483647.0 ? 2147483647 : (int)zr);
 #endif
-  t = t > p.tmin ? t : p.tmin;
-  const uint32_t b2 = (t + 1u) & ~1u;
-  uint32_t no2 = p.negoffs2;
-#if defined(__HIP_DEVICE_COMPILE__)
-  asm("" : "+s"(no2));   // opaque: kept as an addend (one v_lshl_add_u32), not folded into a subtraction
-#endif
-  const uint32_t e2 = (val << 1) + no2;
-  const uint32_t q2 = !(ax < p.thr) ? e2 : b2;
-  const uint32_t s = val >> 31;
+  r = r > p.rmin ? r : p.rmin;
+  const uint32_t q = !(ax < p.thr) ? val + p.negoffs : (uint32_t)r;
+  const uint32_t q2 = q << 1, s = val >> 31;
   return q2 > s ? q2 - s : 0u;   // saturating subtract
 }
 

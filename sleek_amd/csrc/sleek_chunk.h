@@ -923,13 +923,18 @@ __device__ __forceinline__ void decode_chunk_at(const DecodeArgs& a, long long h
       } else {
         int p0 = 8 * (mis + kSub) + sub_bit_offset<W>(L, wv * kWaveRounds);
         const int woff = wv * (kWaveRounds * kSubBytes) + lane * 8;
+        const LossyF32Inv pinv = lossy_f32_inv_params(a.thr_e, (uint32_t)a.offs);   // float32 lossy only
 #pragma unroll
         for (int r = 0; r < kWaveRounds; r++) {
           const int ln = sub_length(L, wv * kWaveRounds + r);
           const uint64_t f = read_field(stage, p0 + lane * PL * ln);
           p0 += ln * Word<W>::PerSub;
           W v[PL];
-          if constexpr (PL == 2) {
+          if constexpr (PL == 2 && LOSSY) {
+            const uint32_t m = ln >= 32 ? 0xffffffffu : ((1u << ln) - 1u);
+            v[0] = lossy_value_f32((uint32_t)f & m, pinv);
+            v[1] = lossy_value_f32((uint32_t)(f >> ln) & m, pinv);
+          } else if constexpr (PL == 2) {
             const uint32_t m = ln >= 32 ? 0xffffffffu : ((1u << ln) - 1u);
             v[0] = from_code<W, LOSSY>((W)f & m, a.thr_e, offs);
             v[1] = from_code<W, LOSSY>((W)(f >> ln) & m, a.thr_e, offs);

@@ -254,7 +254,51 @@ __host__ __device__ __forceinline__ W lossy_value(W enc, int thr_e, W offs)
   const int sh = min(max(clz_w(mag) - (TB - 1 - M), 0), TB - 1);   // in [1, M] where it is used
   const W binned = ((mag << sh) & (((W)1 << M) - 1)) | ((W)(thr_e - sh) << M);
   const W dec = (mag >= ((W)1 << M)) ? (W)(mag + offs) : binned;
-  return (enc == 0) ? (W)0 : (W)(dec | (enc << EM));
+  if constexpr (sizeof(W) == 4) {
+    // mag = 0 for the codes 0 and 2^32 - 1 (enc + 1 wraps, as in the
+    // reference's word type): no magnitude, only the sign bit
+    return (mag == 0) ? (W)(enc << EM) : (W)(dec | (enc << EM));
+  } else {
+    // Deviation: the code 2^64 - 1, which no encoder writes (only a damaged
+    // stream holds it), decodes to ((thr_e - 53) << 52) | 2^63 here and to
+    // the sign bit alone in the reference.  The float64 decoders built from
+    // the select above gave nondeterministic wrong values on the GPU for
+    // valid streams (cause not found), so they keep this form.
+    return (enc == 0) ? (W)0 : (W)(dec | (enc << EM));
+  }
+}
+
+// Float32 inverse of lossy_code: the same function as lossy_value<uint32_t>
+// on all 2^32 codes, in 8 instructions instead of 16.
+//  * Binned values (mag in [1, 2^23)): the reference normalises mag into a
+//    float with exponent field thr_e - sh, sh = 23 - floor(log2 mag).  That
+//    float is mag * 2^(thr_e - 150), a normal number for every thr_e in
+//    [24, 254] (its exponent lies in [thr_e - 150, thr_e - 128]): one
+//    conversion and one multiply, both exact, give its bits.
+//  * mag = 0 (the codes 0 and 2^32 - 1, enc + 1 wrapping as in the
+//    reference): the product is +0, so only the sign bit remains.
+struct LossyF32Inv {
+  float scale;     // 2^(thr_e - 150)
+  uint32_t offs;   // the exact branch's offset
+};
+
+__host__ __device__ __forceinline__ LossyF32Inv lossy_f32_inv_params(int thr_e, uint32_t offs)
+{
+  LossyF32Inv p;
+  const uint32_t sb = (uint32_t)(thr_e - 23) << 23;
+  __builtin_memcpy(&p.scale, &sb, 4);
+  p.offs = offs;
+  return p;
+}
+
+__host__ __device__ __forceinline__ uint32_t lossy_value_f32(uint32_t enc, const LossyF32Inv& p)
+{
+  const uint32_t mag = (enc + 1u) >> 1;
+  const float fb = (float)mag * p.scale;   // exact where it is used (mag < 2^23)
+  uint32_t b;
+  __builtin_memcpy(&b, &fb, 4);
+  const uint32_t dec = mag >= (1u << 23) ? mag + p.offs : b;
+  return dec | (enc << 31);
 }
 
 template <typename W, bool LOSSY>

@@ -1,9 +1,10 @@
-// On the GPU: the float form of the float32 error-bounded code
-// (lossy_code_f32, the one the encoders run) against the integer form
-// (lossy_code<uint32_t>, equal to the oracle's quantize on every pattern,
-// checked on the host by transform_check) over all 2^32 float32 bit patterns,
-// at bounds of every class.  This pins the float form on the hardware itself
-// (its denormal handling and float-to-integer conversion).
+// On the GPU: the float forms of the float32 error-bounded code and of its
+// inverse (lossy_code_f32 and lossy_value_f32, the ones the kernels run)
+// against the integer forms (lossy_code<uint32_t> and lossy_value<uint32_t>,
+// equal to the oracle's quantize and dequantize on every pattern and code,
+// checked on the host by transform_check) over all 2^32 float32 bit patterns
+// (each also taken as a code), at bounds of every class.  This pins the float
+// forms on the hardware itself (its denormal handling and its conversions).
 // Prints one line per bound; exit status 0 = no mismatch.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -20,13 +21,16 @@ __global__ __launch_bounds__(kThreadsPerBlock) void check_kernel(int eb_e, int t
                                                                  unsigned long long* bad, unsigned* first)
 {
   const sleek::LossyF32 p = sleek::lossy_f32_params(thr_e, offs);
+  const sleek::LossyF32Inv pi = sleek::lossy_f32_inv_params(thr_e, offs);
   const uint64_t t = (uint64_t)blockIdx.x * kThreadsPerBlock + threadIdx.x;
   const uint32_t x0 = (uint32_t)(t * kPer);
   unsigned long long n = 0;
   for (uint32_t i = 0; i < kPer; i++) {
     const uint32_t x = x0 + i;
     const uint32_t a = sleek::lossy_code_f32(x, p), b = sleek::lossy_code<uint32_t>(x, eb_e, thr_e, offs);
-    if (a != b) {
+    // x also as a code: the float inverse against the integer one
+    const uint32_t c = sleek::lossy_value_f32(x, pi), d = sleek::lossy_value<uint32_t>(x, thr_e, offs);
+    if (a != b || c != d) {
       n++;
       atomicMin(first, x);
     }

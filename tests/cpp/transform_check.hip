@@ -1,8 +1,10 @@
 // Host-side check of the codec's value transforms (sleek_amd/csrc/sleek_device.h,
 // the very functions the kernels inline) against the oracle: lossless code
-// and inverse, and error-bounded code (both forms, integer and float) and
-// inverse at bounds of every class (denormal, tiny, ordinary, the largest
-// allowed), over all 2^32 float32 bit patterns and over random float64 patterns covering every exponent.
+// and inverse, and error-bounded code and inverse (both forms of each,
+// integer and float; the inverse also on every 32-bit code) at bounds of
+// every class (denormal, tiny, ordinary, the largest allowed), over all 2^32
+// float32 bit patterns and over random float64 patterns covering every
+// exponent.
 // Exit status 0 = no mismatch.
 #include <stdint.h>
 #include <stdio.h>
@@ -47,6 +49,10 @@ int main()
           b += (qc != sleek_oracle_quantize_u32(x, q.e, q.t, q.o));
           b += (sleek::lossy_code_f32(x, sleek::lossy_f32_params(q.t, q.o)) != qc);   // the float form
           b += (sleek::lossy_value<uint32_t>(qc, q.t, q.o) != sleek_oracle_dequantize_u32(qc, q.t, q.o));
+          // x taken as a code: every code, valid or not, both inverse forms against the oracle
+          const uint32_t dx = sleek_oracle_dequantize_u32(x, q.t, q.o);
+          b += (sleek::lossy_value<uint32_t>(x, q.t, q.o) != dx);
+          b += (sleek::lossy_value_f32(x, sleek::lossy_f32_inv_params(q.t, q.o)) != dx);   // the float form
         }
       }
       uint64_t s = 0x9E3779B97F4A7C15ull * (w + 1);
@@ -62,6 +68,8 @@ int main()
           const uint64_t qc = sleek::lossy_code<uint64_t>(x, q.e, q.t, q.o);
           b += (qc != sleek_oracle_quantize_u64(x, q.e, q.t, q.o));
           b += (sleek::lossy_value<uint64_t>(qc, q.t, q.o) != sleek_oracle_dequantize_u64(qc, q.t, q.o));
+          if (x != ~0ull)   // any code but 2^64 - 1, whose decoding deviates (see lossy_value)
+            b += (sleek::lossy_value<uint64_t>(x, q.t, q.o) != sleek_oracle_dequantize_u64(x, q.t, q.o));
         }
       }
       bad += b;

@@ -347,6 +347,46 @@ def test_corrupt_stream_detected(gpu):
     assert d.cpu().numpy().tobytes() == a.tobytes()
 
 
+def edge_code_stream(dt, seed=7):
+    """A one-chunk lossy stream, well formed, whose first subchunk is a full
+    word wide and holds the edge codes (0, 1, 2, the largest codes and, for
+    float32, the code whose enc + 1 wraps) and random ones; the other
+    subchunks are 0 bits wide.  No encoder writes the largest codes, but a
+    decoder meets them in damaged streams and must map them as the reference
+    does (sleek_common_gpu.h:1077-1108).  The float64 decoder deviates on the
+    wrapping code 2^64 - 1 (lossy_value in sleek_device.h), which is left out."""
+    wt = np.uint32 if dt == "f32" else np.uint64
+    tb = 32 if dt == "f32" else 64
+    per_sub = 128 if dt == "f32" else 64
+    top = wt(np.iinfo(wt).max)
+    codes = np.random.default_rng(seed).integers(0, np.iinfo(wt).max, per_sub, dtype=wt, endpoint=False)
+    codes[:8] = [0, 1, 2, 3, top if dt == "f32" else top - wt(3), top - wt(1), top - wt(2), top >> wt(1)]
+    widths = np.zeros(32, np.uint8)
+    widths[0] = tb
+    payload = widths.tobytes() + codes.tobytes() + np.int16(16384).tobytes()
+    return np.frombuffer(np.int64(16384).tobytes() + np.uint16(len(payload)).tobytes() + payload, np.uint8).copy()
+
+
+@pytest.mark.parametrize("dt,eb", [("f32", 1e-3), ("f32", 1e-40), ("f32", 9e30), ("f64", 1e-6), ("f64", 1e-310)])
+def test_lossy_decode_of_edge_codes(gpu, dt, eb):
+    """Every code decodes as the reference's deQuantize does, the ones that
+    only damaged streams hold included."""
+    import torch
+    from sleek_amd import codec
+
+    say(f"edge codes {dt} eb={eb}")
+    stream = edge_code_stream(dt)
+    dtype = np.float32 if dt == "f32" else np.float64
+    want, st = po.decompress(stream, dtype, eb)
+    assert st == 0
+    tdt = torch.float32 if dt == "f32" else torch.float64
+    ws = codec.Workspace(16384)
+    d, dn = codec.decompress(torch.from_numpy(stream).cuda(), tdt, eb, numel=16384 // np.dtype(dtype).itemsize,
+                             workspace=ws, enclen=stream.size)
+    assert ws.status() == 0 and int(dn.item()) == 16384
+    assert d.cpu().numpy().tobytes() == want.tobytes()
+
+
 @pytest.mark.parametrize("dt,n,eb", [("f32", 100_003, 0.0), ("f32", 100_003, 1e-3), ("f64", 50_001, 0.0),
                                      ("f64", 50_001, 1e-6), ("f32", 0, 0.0)])
 def test_cpp_dropin_headers(gpu, tmp_path, dt, n, eb):

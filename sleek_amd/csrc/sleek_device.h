@@ -595,6 +595,44 @@ __device__ __forceinline__ void or_codes(uint32_t* st, int p0, int lane, int ln,
   or_field(st, p0 + lane * ln, v[0]);
 }
 
+// A lane's codes of one subchunk into the LDS bit buffer, with no clearing
+// beforehand.  The subchunk's fields start at bit p0 (a multiple of 32) and
+// cover its dwords exactly: lane l's field of w = PL * ln bits sits at bit
+// p0 + x, x = l * w.  Every one of those dwords has its bit 0 inside exactly
+// one lane's field; that lane writes the dword whole (its field's bits from
+// there up, zeros above them).  Then each lane whose field starts inside a
+// dword ORs its first bits into it.  A wave's LDS operations execute in
+// order, and the compiler barrier keeps the writes ahead of the ORs, so
+// every OR lands on its dword's write: one conflict-free write per dword and
+// at most one OR per lane, where OR-ing into a cleared buffer took a clearing
+// pass and two or three ORs per lane.
+__device__ __forceinline__ void put_field(uint32_t* st, int p0, uint32_t x, uint32_t w, uint64_t f)
+{
+  const uint32_t off = x & 31u, s1 = (0u - x) & 31u;   // s1: bits from the field to the next dword start
+  uint32_t* const dw = st + (p0 >> 5) + ((x + 31u) >> 5);
+  if (s1 < w) dw[0] = (uint32_t)(f >> s1);
+  if (s1 + 32u < w) dw[1] = (uint32_t)(f >> (s1 + 32u));
+  asm volatile("" ::: "memory");
+  if (off) atomicOr(st + (p0 >> 5) + (x >> 5), (uint32_t)(f << off));
+}
+__device__ __forceinline__ void put_codes(uint32_t* st, int p0, int lane, int ln, const uint32_t v[2])
+{
+  const uint32_t w = (uint32_t)(2 * ln), x = (uint32_t)lane * w;
+  if (ln <= 16) {   // uniform: a field of at most 32 bits
+    const uint32_t off = x & 31u, s1 = (0u - x) & 31u;
+    const uint32_t f = v[0] | (v[1] << ln);
+    if (s1 < w) st[(p0 >> 5) + ((x + 31u) >> 5)] = f >> s1;
+    asm volatile("" ::: "memory");
+    if (off) atomicOr(st + (p0 >> 5) + (x >> 5), f << off);
+  } else {
+    put_field(st, p0, x, w, (uint64_t)v[0] | ((uint64_t)v[1] << ln));
+  }
+}
+__device__ __forceinline__ void put_codes(uint32_t* st, int p0, int lane, int ln, const uint64_t v[1])
+{
+  put_field(st, p0, (uint32_t)(lane * ln), (uint32_t)ln, v[0]);
+}
+
 // The 64 bits of an LDS bit buffer starting at bit p.
 __device__ __forceinline__ uint64_t read_field(const uint32_t* st, int p)
 {

@@ -717,9 +717,34 @@ __device__ __forceinline__ void pack_half(const EncodeArgs& a, long long c, int 
   }
 }
 
+// The lines of pair_store for a head whose dword part is Q (uniform, so the
+// dword selection is static): lines j = 64 h + lane + 128 i, two per trip
+// with their four 128-bit reads waited for together.
+template <int Q>
+__device__ __forceinline__ void pair_store_lines(uint8_t* dst, uint32_t lds_base, int head, int nfull, int h,
+                                                 uint32_t sh)
+{
+  const int lane = threadIdx.x & 63;
+  int j = h * 64 + lane;
+  for (; j + 128 < nfull; j += 256) {
+    u32x4 a0, b0, a1, b1;
+    lds_read_lines4(lds_base + (uint32_t)(j << 4), lds_base + (uint32_t)((j + 128) << 4), a0, b0, a1, b1);
+    gstore((uint4*)(dst + head + (j << 4)), line_window<Q>(a0, b0, sh));
+    gstore((uint4*)(dst + head + ((j + 128) << 4)), line_window<Q>(a1, b1, sh));
+  }
+  if (j < nfull) {
+    u32x4 a0, b0;
+    lds_read_lines2(lds_base + (uint32_t)(j << 4), a0, b0);
+    gstore((uint4*)(dst + head + (j << 4)), line_window<Q>(a0, b0, sh));
+  }
+}
+
 // len bytes of the staging buffer to dst (any byte address) by both waves:
 // 16-byte lines alternate between the waves, wave 0 stores the head bytes
-// before the first 16-byte boundary, wave 1 the tail bytes.
+// before the first 16-byte boundary, wave 1 the tail bytes.  Line j of the
+// output is staging bytes head + 16 j .. + 15, inside staging lines j and
+// j + 1 (head < 16), which are read whole: reads of single dwords 16 bytes
+// apart across the wave would conflict 8-way.
 __device__ __forceinline__ void pair_store(uint8_t* dst, const uint32_t* lds, int len, int h)
 {
   const int lane = threadIdx.x & 63;
@@ -728,9 +753,13 @@ __device__ __forceinline__ void pair_store(uint8_t* dst, const uint32_t* lds, in
   const int nfull = (len - head) >> 4;
   const int tail = len - head - (nfull << 4);
   const uint8_t* lb = (const uint8_t*)lds;
-  for (int j = h * 64 + lane; j < nfull; j += 128) {
-    const int p = head + (j << 4);
-    gstore((uint4*)(dst + p), lds_read16(lds, p));
+  const uint32_t base = lds_addr(lds);
+  const uint32_t sh = (uint32_t)(head & 3) * 8u;
+  switch (uni(head >> 2)) {
+    case 0: pair_store_lines<0>(dst, base, head, nfull, h, sh); break;
+    case 1: pair_store_lines<1>(dst, base, head, nfull, h, sh); break;
+    case 2: pair_store_lines<2>(dst, base, head, nfull, h, sh); break;
+    default: pair_store_lines<3>(dst, base, head, nfull, h, sh); break;
   }
   // the edge bytes (disjoint from the lines: any order is correct)
   if (h == 0 && lane < head) gstore(dst + lane, lb[lane]);

@@ -730,6 +730,42 @@ __device__ __forceinline__ uint4 lds_read16(const uint32_t* lds, int byte_off)
                     __builtin_amdgcn_alignbit(x3, x2, sh), __builtin_amdgcn_alignbit(x4, x3, sh));
 }
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// The LDS byte address of a __shared__ buffer passed as a generic pointer.
+__device__ __forceinline__ uint32_t lds_addr(const void* p)
+{
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+
+// Whole 16-byte LDS lines at byte addresses a, a + 16 (and c, c + 16):
+// 128-bit reads, which the compiler would otherwise narrow to the dwords
+// used (bank conflicts across the wave); the wait for them is in the same
+// asm block, so the outputs are complete when it ends.
+__device__ __forceinline__ void lds_read_lines2(uint32_t a, u32x4& x, u32x4& y)
+{
+  asm volatile("ds_read_b128 %0, %2\n\tds_read_b128 %1, %2 offset:16\n\ts_waitcnt lgkmcnt(0)"
+               : "=&v"(x), "=&v"(y)
+               : "v"(a));
+}
+__device__ __forceinline__ void lds_read_lines4(uint32_t a, uint32_t c, u32x4& x, u32x4& y, u32x4& z, u32x4& w)
+{
+  asm volatile("ds_read_b128 %0, %4\n\tds_read_b128 %1, %4 offset:16\n\t"
+               "ds_read_b128 %2, %5\n\tds_read_b128 %3, %5 offset:16\n\ts_waitcnt lgkmcnt(0)"
+               : "=&v"(x), "=&v"(y), "=&v"(z), "=&v"(w)
+               : "v"(a), "v"(c));
+}
+
+// 16 bytes from a window of two consecutive lines (a, b): dwords Q .. Q + 4
+// shifted right by sh bits (sh = 8 * (byte offset mod 4)).
+template <int Q>
+__device__ __forceinline__ uint4 line_window(const u32x4& a, const u32x4& b, uint32_t sh)
+{
+  const uint32_t d[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+  return make_uint4(__builtin_amdgcn_alignbit(d[Q + 1], d[Q], sh), __builtin_amdgcn_alignbit(d[Q + 2], d[Q + 1], sh),
+                    __builtin_amdgcn_alignbit(d[Q + 3], d[Q + 2], sh), __builtin_amdgcn_alignbit(d[Q + 4], d[Q + 3], sh));
+}
+
 // Copy len bytes from LDS to global memory at an arbitrary byte address, by
 // one wave (the single-wave variant of store_unaligned below).
 __device__ __forceinline__ void wave_store_unaligned(uint8_t* dst, const uint32_t* lds, int len)

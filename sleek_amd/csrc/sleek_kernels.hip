@@ -370,6 +370,12 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
   __shared__ long long xnext, xfirst;
   const int h = uni((int)(threadIdx.x >> 6));   // wave-uniform: scalar branches and bit lengths
   const int lane = threadIdx.x & 63;
+  // Where wave 0 publishes chunk c's inclusive word: before the store (the
+  // compiler then drains it, with every other memory operation in flight,
+  // before the store loop) or after it (later for the chunks that wait on
+  // it).  After is faster for the float32 lossy encoders' short stores,
+  // before for the others (measured).
+  constexpr bool kLatePublish = LOSSY && sizeof(W) == 4;
   const long long nchunks = (a.insize + kChunk - 1) / kChunk;
   if (nchunks == 0) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -436,7 +442,7 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
       long long m = -1;
       if (TICKET && more) m = ticket_chunk(a.tickets, a.ticket_shift, tq, nchunks);
       if (lane == 0) {
-        if (c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)sc.csize);
+        if (!kLatePublish && c != 0) publish_inclusive(st, c, nchunks, excl + (uint64_t)sc.csize);
         xexcl = excl;
         if (TICKET) xnext = m;
         if (c == 0) *(long long*)a.out = a.insize;
@@ -447,6 +453,8 @@ __global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 4))) voi
     const long long excl = (long long)uni((long long)xexcl);
     const long long nn = TICKET ? uni(xnext) : n + gridDim.x;
     pair_store(a.out + data_base + excl, stage, sc.csize, h);
+    if (kLatePublish && h == 0 && lane == 0 && c != 0)
+      publish_inclusive(st, c, nchunks, (uint64_t)excl + (uint64_t)sc.csize);
     __syncthreads();   // staging buffer read before it is packed again
     if (!more) break;
     c = n;

@@ -902,9 +902,11 @@ __device__ __forceinline__ void scan_offsets(const DecodeArgs& a, long long* wsu
 // loads of the payload's cover into LDS -> per-lane bit extraction ->
 // inverse transform -> coalesced stores.  hdr / nchunks come from a header
 // that passed header_ok.  Damage found is reported in the status word.
+// csize_known: the chunk's size-table entry when the caller has it (from
+// the offsets table), else -1 and it is read here.
 template <typename W, bool LOSSY>
 __device__ __forceinline__ void decode_chunk_at(const DecodeArgs& a, long long hdr, long long nchunks, long long c,
-                                                long long excl, uint32_t* stage)
+                                                long long excl, uint32_t* stage, int csize_known = -1)
 {
   constexpr int PL = Word<W>::PerLane;
   constexpr int TB = Word<W>::TB;
@@ -916,7 +918,7 @@ __device__ __forceinline__ void decode_chunk_at(const DecodeArgs& a, long long h
   const W offs = (W)a.offs;
   const long long base = c * kChunk;
   const int osize = (int)min((long long)kChunk, hdr - base);
-  const int csize = uni((int)sizes[c]);
+  const int csize = csize_known >= 0 ? csize_known : uni((int)sizes[c]);
   const bool bad = csize > osize || (csize < osize && csize < kSub + 2) ||
                    (a.inlen >= 0 && data_base + excl + csize > a.inlen);   // payload beyond the stream
 

@@ -512,16 +512,34 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
   }
 }
 
-// Decoder: workgroups stride over the chunks in any order.
+// Decoder: workgroups stride over the chunks in any order.  A chunk's
+// offset and size come from the offsets table (offs[c] and offs[c + 1], the
+// offsets kernel having written entries 0 .. nchunks), loaded together with
+// the stream header: one round trip before the payload's instead of two.
+// The table is read before the header is checked, at indexes below the
+// grid's chunk count plus one, which the workspace (sized for the capacity's
+// chunks, as is the grid) holds; entries beyond the stream's chunks are not
+// used.
 template <typename W, bool LOSSY>
 __global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
-  const long long hdr = uni(*(const long long*)a.in);
+  const long long* const offs_tab = (const long long*)(a.ws + kWsFlags);
+  long long c = blockIdx.x;
+  const long long hdr_raw = *(const long long*)a.in;
+  long long o0 = offs_tab[c], o1 = offs_tab[c + 1];
+  const long long hdr = uni(hdr_raw);
   long long nchunks = 0;
   uint64_t why = 0;
   if (!header_ok(a, hdr, &nchunks, &why)) return;   // reported by offsets_kernel
-  for (long long c = blockIdx.x; c < nchunks; c += gridDim.x) decode_chunk<W, LOSSY>(a, hdr, nchunks, c, stage);
+  for (; c < nchunks; c += gridDim.x) {
+    if (c != (long long)blockIdx.x) {
+      o0 = offs_tab[c];
+      o1 = offs_tab[c + 1];
+    }
+    const long long excl = uni(o0);
+    decode_chunk_at<W, LOSSY>(a, hdr, nchunks, c, excl, stage, (int)(uni(o1) - excl));
+  }
 }
 
 // ---- launchers ----

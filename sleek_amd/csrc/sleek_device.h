@@ -738,6 +738,20 @@ __device__ __forceinline__ uint32_t lds_addr(const void* p)
   return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
 }
 
+// Direct global -> LDS load of 16 bytes per lane (`global_load_lds_dwordx4`,
+// gfx950): lane i's 16 bytes at src land at LDS byte wave_base + 16 i, with
+// wave_base (the LDS pointer, taken through M0) the same for the whole wave;
+// lanes switched off write nothing.  No registers hold the data, so a lane
+// can have all its loads in flight at once; the loads count in vmcnt, and
+// the LDS holds their bytes once a wave has waited for vmcnt(0).  NT sets
+// the non-temporal hint (see ntload).
+template <bool NT>
+__device__ __forceinline__ void load_to_lds16(const void* src, void* wave_base)
+{
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                   (__attribute__((address_space(3))) void*)wave_base, 16, 0, NT ? 2 : 0);
+}
+
 // Whole 16-byte LDS lines at byte addresses a, a + 16 (and c, c + 16):
 // 128-bit reads, which the compiler would otherwise narrow to the dwords
 // used (bank conflicts across the wave); the wait for them is in the same

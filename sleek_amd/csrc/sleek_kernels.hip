@@ -521,7 +521,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8, 8))
 // chunks, as is the grid) holds; entries beyond the stream's chunks are not
 // used.
 template <typename W, bool LOSSY>
-__global__ __launch_bounds__(kThreads) void decode_kernel(DecodeArgs a)
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(8))) void decode_kernel(DecodeArgs a)
 {
   __shared__ __align__(16) uint32_t stage[kStageBytes / 4];
   const long long* const offs_tab = (const long long*)(a.ws + kWsFlags);

@@ -929,18 +929,19 @@ __device__ __forceinline__ void decode_chunk_at(const DecodeArgs& a, long long h
     const uint4* const cover = (const uint4*)(src - mis);
     const int nblk = (mis + csize + 15) >> 4;
     // streamed payload reads for lossless streams, streamed output stores for
-    // lossy ones (each measured best for its mode; see ntload).  A lossless
-    // cover (up to 16 KiB) goes straight to LDS, all of a lane's loads in
-    // flight together: block j = t + k * kThreads, which wave wv's k-th load
-    // writes at its base stage block k * kThreads + 64 wv plus its lane.  A
-    // lossy cover, a few KiB, takes one or two trips of the plain loop.
-    if constexpr (!LOSSY) {
+    // lossy ones (each measured best for its mode; see ntload).  A cover of
+    // more than one block per thread (every lossless one, up to 16 KiB) goes
+    // straight to LDS, all of a lane's loads in flight together: block
+    // j = t + k * kThreads, which wave wv's k-th load writes at its base
+    // stage block k * kThreads + 64 wv plus its lane.  A smaller lossy cover
+    // takes one trip of the plain loop.
+    if (!LOSSY || nblk > kThreads) {
       constexpr int kCoverLoads = (kStageBytes / 16 + kThreads - 1) / kThreads;
       static_assert((15 + kChunk + 15) / 16 <= kStageBytes / 16, "the cover fits the stage");
 #pragma unroll
       for (int k = 0; k < kCoverLoads; k++) {
         const int j = t + k * kThreads;
-        if (j < nblk) load_to_lds16<true>(cover + j, (uint4*)stage + (k * kThreads + 64 * wv));
+        if (j < nblk) load_to_lds16<!LOSSY>(cover + j, (uint4*)stage + (k * kThreads + 64 * wv));
       }
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     } else {

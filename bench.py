@@ -622,7 +622,9 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this codec runs on one GPU; multi-GPU runs are out of scope "
-                                   "(the sharding design is described in DESIGN.md)", "n_gpus": args.gpus}))
+                                   "(a field would shard into contiguous runs of whole chunks, one per GPU, "
+                                   "with one exchange of the runs' compressed sizes)",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
     import torch
 
